@@ -1,0 +1,222 @@
+/*
+ * fluidsim.h -- C ABI of the MI355X (gfx950) D3Q19 lattice-Boltzmann step and
+ * STL voxelizer (libfluidsim.so).
+ *
+ * Plain C: pointers, sizes and int status codes; no C++ or torch types cross
+ * this boundary and no C++ exception escapes it.  Every fs_* function returns
+ * FS_OK (0) or a negative FS_ERR_* code; fs_last_error() then holds a message
+ * (per calling thread).
+ *
+ * Each declaration cites the reference interface it replaces
+ * (yagise/MyFluidSimulation, file:line).  The reference exposes a duck-typed
+ * C++ "Solver" class per method (LBM3D_Legacy / LBM3D_Home / LBM3D_Hybrid);
+ * fs_solver is that class behind an opaque handle, with the method chosen at
+ * creation.  Device pointers handed out by a solver are owned by it and stay
+ * valid until fs_solver_destroy (HOME's macro pointers move on every step, as
+ * in the reference).  Memory passed in by callers is never freed by the
+ * library.
+ *
+ * Threading: a solver may be used from one host thread at a time.  Work runs
+ * on the solver's own HIP stream (default priority); the synchronous calls
+ * wait for it before returning, as every reference solver method ends with
+ * cudaDeviceSynchronize (lbm3d_legacy.cu:158,207,223).
+ */
+#ifndef FLUIDSIM_H
+#define FLUIDSIM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FS_ABI_VERSION 1
+
+/* ---- status codes ---- */
+#define FS_OK 0
+#define FS_ERR_INVALID (-1) /* bad argument or flag grid */
+#define FS_ERR_DEVICE (-2)  /* HIP runtime error (no GPU, launch failure) */
+#define FS_ERR_ALLOC (-3)   /* device or host allocation failed */
+#define FS_ERR_IO (-4)      /* file could not be opened, read or written */
+#define FS_ERR_STATE (-5)   /* call not valid for this solver's method */
+
+/* ---- methods: lbm3d_legacy.hpp:19, lbm3d_home.hpp:15, lbm3d_hybrid.hpp:18 ---- */
+#define FS_METHOD_BGK 0    /* LBM3D_Legacy: BGK collide + stream */
+#define FS_METHOD_HOME 1   /* LBM3D_Home: 10 moments per cell */
+#define FS_METHOD_HYBRID 2 /* LBM3D_Hybrid: BGK band + two-rate bulk */
+
+/* ---- per-cell flag byte ---- */
+#define FS_FLAG_SOLID 0x01u  /* solid (reference solid mask, lbm3d_legacy.cu:72) */
+#define FS_FLAG_BAND 0x02u   /* Hybrid BGK band (reference isLegacy, lbm3d_hybrid.cu:137) */
+#define FS_FLAG_INLET 0x04u  /* extension: equilibrium inlet cell */
+#define FS_FLAG_OUTLET 0x08u /* extension: zero-gradient outlet cell (x = nx-1) */
+
+/* ---- macro selectors for fs_solver_macro_ptr / fs_solver_copy_macro ---- */
+#define FS_RHO 0
+#define FS_UX 1
+#define FS_UY 2
+#define FS_UZ 3
+
+typedef struct fs_solver fs_solver;
+
+/* ABI version (FS_ABI_VERSION) and the last error message of this thread. */
+int fs_version(void);
+const char* fs_last_error(void);
+/* Number of visible HIP devices (0 when none); never fails on a CPU host. */
+int fs_device_count(int* count);
+
+/* ===================== solver (the reference Solver concept) ============ */
+
+/* LBM3D_*::init(const Domain&) (lbm3d_legacy.cu:141-147, lbm3d_home.cu:301-307,
+ * lbm3d_hybrid.cu:210-221) with Domain{Nx,Ny,Nz,tau,forceX..Z}
+ * (lbm_common.hpp:6-10): allocates device state and resets it to rho=1,u=0. */
+int fs_solver_create(int method, int nx, int ny, int nz, float tau, float fx,
+                     float fy, float fz, fs_solver** out);
+/* ~LBM3D_*() (lbm3d_legacy.cu:14): frees every device buffer. */
+int fs_solver_destroy(fs_solver* s);
+/* Nx() Ny() Nz() N() (lbm3d_legacy.hpp:60-64); any pointer may be NULL. */
+int fs_solver_dims(const fs_solver* s, int* nx, int* ny, int* nz, size_t* n);
+int fs_solver_method(const fs_solver* s, int* method);
+
+/* setSolidMask(const unsigned char*) (lbm3d_legacy.cu:149-151): N host bytes,
+ * nonzero = solid.  Clears any inlet/outlet flags; keeps the band. */
+int fs_solver_set_solid_mask(fs_solver* s, const uint8_t* host_mask);
+/* LBM3D_Hybrid::setLegacyMapping(h_isLegacy, h_legacySlot_ignored)
+ * (lbm3d_hybrid.cu:227-235): N host bytes or NULL (= no band cell); the slot
+ * array is ignored, as in the reference.  FS_ERR_STATE for other methods. */
+int fs_solver_set_legacy_mapping(fs_solver* s, const uint8_t* host_is_legacy,
+                                 const int* ignored_slots);
+/* Extension: the whole flag byte per cell (FS_FLAG_*).  INLET cells must not
+ * be SOLID; OUTLET cells must sit at x >= 1 with a fluid, non-inlet,
+ * non-outlet upstream neighbour x-1.  FS_ERR_INVALID otherwise. */
+int fs_solver_set_flags(fs_solver* s, const uint8_t* host_flags);
+int fs_solver_copy_flags(fs_solver* s, uint8_t* host_flags);
+/* Extension: inlet x-velocity per (y,z) (ny*nz floats, index z*ny+y); NULL
+ * sets it to zero.  Host pointer. */
+int fs_solver_set_inlet_profile(fs_solver* s, const float* host_ux);
+/* setForce(fx,fy,fz) (lbm3d_home.hpp:33, lbm3d_hybrid.hpp:39). */
+int fs_solver_set_force(fs_solver* s, float fx, float fy, float fz);
+/* Hybrid relaxation rates (lbm3d_hybrid.cu:264-267 hard-wires all three to
+ * 1/tau, which is the default here).  FS_ERR_STATE for other methods. */
+int fs_solver_set_omegas(fs_solver* s, float omega_legacy, float omega_hydro,
+                         float omega_shear);
+
+/* reset() (lbm3d_legacy.cu:153-159, lbm3d_home.cu:313-319,
+ * lbm3d_hybrid.cu:237-243). */
+int fs_solver_reset(fs_solver* s);
+/* reinitEquilibriumFromMacro(d_rho,d_ux,d_uy,d_uz) (lbm3d_legacy.cu:160-208,
+ * lbm3d_home.cu:321-333, lbm3d_hybrid.cu:245-257): device pointers of N
+ * floats each, any may be NULL with each method's reference meaning. */
+int fs_solver_reinit_from_macro(fs_solver* s, const float* d_rho,
+                                const float* d_ux, const float* d_uy,
+                                const float* d_uz);
+/* step(int substeps) (lbm3d_legacy.cu:210-224, lbm3d_home.cu:335-354,
+ * lbm3d_hybrid.cu:259-283): nsteps time steps; returns when they are done. */
+int fs_solver_step(fs_solver* s, int nsteps);
+/* Extension: enqueue nsteps on the solver's stream and return at once. */
+int fs_solver_step_async(fs_solver* s, int nsteps);
+/* Extension: stream handling.  set_stream(NULL) restores the solver's own
+ * stream; the caller keeps ownership of a stream it passes in. */
+int fs_solver_set_stream(fs_solver* s, void* hip_stream);
+int fs_solver_get_stream(const fs_solver* s, void** hip_stream);
+int fs_solver_synchronize(fs_solver* s);
+/* Extension: enqueue nsteps between two HIP events recorded on the solver's
+ * stream, wait, and return the elapsed device time in milliseconds. */
+int fs_solver_timed_steps(fs_solver* s, int nsteps, float* elapsed_ms);
+
+/* d_rho() d_u() d_v() d_w() (lbm3d_legacy.cu:229-236, lbm3d_home.cu:355-362):
+ * borrowed device pointer to N floats.  BGK and Hybrid report the moments of
+ * the populations at the start of the last step (the reference's one-step
+ * lag); HOME reports its current moments. */
+int fs_solver_macro_ptr(fs_solver* s, int which, float** d_ptr);
+int fs_solver_copy_macro(fs_solver* s, int which, float* host_dst);
+
+/* Extension: raw state in the reference layout: populations f[q*N+id]
+ * (19N floats, BGK/Hybrid; the incoming populations of the current time
+ * level, as the reference's d_f() holds after step) or moments m[k*N+id]
+ * (10N floats, HOME; k = rho,ux,uy,uz,Sxx,Sxy,Sxz,Syy,Syz,Szz). */
+int fs_solver_state_size(const fs_solver* s, size_t* nfloats);
+int fs_solver_copy_state(fs_solver* s, float* host_dst);
+int fs_solver_load_state(fs_solver* s, const float* host_src);
+
+/* ===================== geometry ========================================= */
+
+/* load_stl(path, TriangleMesh&) (stl_loader.cpp:102-122): triangle soup,
+ * 9 floats per triangle (v0,v1,v2), and the vertex bounding box.  Call with
+ * tri_xyz == NULL to get the count; FS_ERR_IO when the file cannot be read
+ * or parsed (the reference's `false`). */
+int fs_load_stl(const char* path, float* tri_xyz, int capacity, int* ntri,
+                float* bbmin3, float* bbmax3);
+/* voxelize_mesh_to_mask(mesh, VoxelParams) (voxelize.cpp:228-243,
+ * VoxelParams voxelize.hpp:16-24), on the GPU: nx*ny*nz host bytes out,
+ * 1 = solid, x fastest. */
+int fs_voxelize(const float* host_tri_xyz, int ntri, const float* bbmin3,
+                const float* bbmax3, int nx, int ny, int nz,
+                float uniform_scale, float tx, float ty, float tz, int refine,
+                float threshold, uint8_t* host_mask_out);
+/* build_distance_to_solid_6n + build_legacy_mask_from_distance
+ * (sim_runner.hpp:28-91, maxDist = max(1,d0) as at sim_runner.hpp:222), on
+ * the GPU: band = fluid cells within 6-neighbour distance d0 of a solid. */
+int fs_band(const uint8_t* host_solid, int nx, int ny, int nz, int d0,
+            uint8_t* host_band_out);
+
+/* ===================== fields and output ================================ */
+
+/* launch_speed (field_ops.cu:25-28) */
+int fs_speed(const float* d_u, const float* d_v, const float* d_w,
+             float* d_out, size_t n);
+/* make_rho_gauss / make_rho_slab (init_profiles.cu:31-45) */
+int fs_rho_gauss(float* d_rho, int nx, int ny, int nz, float cx, float cy,
+                 float cz, float sigma, float amp);
+int fs_rho_slab(float* d_rho, int nx, int ny, int nz, int axis, float amp,
+                float center, float width);
+/* reinit_equilibrium_from_macro (lbm_reinit.cu:52-61): f[q*n+i] = feq */
+int fs_reinit_equilibrium(const float* d_rho, const float* d_ux,
+                          const float* d_uy, const float* d_uz, float* d_f,
+                          size_t n);
+/* write_scalar_vtk / write_vector_vtk (moment_vtk_writer.cu:9-49): device
+ * field(s) of nx*ny*nz floats to a legacy ASCII STRUCTURED_POINTS file. */
+int fs_write_scalar_vtk(const char* filename, const float* d_field, int nx,
+                        int ny, int nz);
+int fs_write_vector_vtk(const char* filename, const float* d_fx,
+                        const float* d_fy, const float* d_fz, int nx, int ny,
+                        int nz);
+
+/* Device memory helpers for callers built without HIP headers. */
+int fs_device_malloc(void** d_ptr, size_t bytes);
+int fs_device_free(void* d_ptr);
+int fs_memcpy_htod(void* d_dst, const void* h_src, size_t bytes);
+int fs_memcpy_dtoh(void* h_dst, const void* d_src, size_t bytes);
+int fs_memset_device(void* d_dst, int value, size_t bytes);
+int fs_device_synchronize(void);
+/* Bandwidth probe: `iters` launches of a streaming copy of `bytes` bytes
+ * (src -> dst, both device, 16-byte aligned, bytes % 16 == 0), timed with
+ * HIP events; *elapsed_ms is the total.  Gives the measured copy ceiling. */
+int fs_copy_probe(void* d_dst, const void* d_src, size_t bytes, int iters, float* elapsed_ms);
+
+/* ===================== reference-named entry points ===================== */
+/* Same names and signatures as the reference's extern "C" helpers
+ * (declared by its consumers at sim_common.hpp:30-46, sim_runner.hpp:24).
+ * Like the reference they return nothing; on failure they print the error
+ * to stderr and abort, as the reference's throw across extern "C" ends the
+ * process (moment_vtk_writer.cu:14,38). */
+void write_scalar_vtk(const char* filename, const float* d_field, int Nx,
+                      int Ny, int Nz);
+void write_vector_vtk(const char* filename, const float* d_fx,
+                      const float* d_fy, const float* d_fz, int Nx, int Ny,
+                      int Nz);
+void make_rho_gauss(float* d_rho, int Nx, int Ny, int Nz, float cx, float cy,
+                    float cz, float sigma, float amp);
+void make_rho_slab(float* d_rho, int Nx, int Ny, int Nz, int axis, float amp,
+                   float center, float width);
+void reinit_equilibrium_from_macro(const float* d_rho, const float* d_ux,
+                                   const float* d_uy, const float* d_uz,
+                                   float* d_f, int N);
+void launch_speed(const float* u, const float* v, const float* w, float* out,
+                  int N);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FLUIDSIM_H */

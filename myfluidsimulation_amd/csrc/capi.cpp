@@ -1,0 +1,744 @@
+// capi.cpp -- the C ABI declared in include/fluidsim.h.
+//
+// Owns every device buffer of a solver, validates arguments before touching
+// the device, converts HIP errors and C++ exceptions into status codes and a
+// per-thread message, and sequences the kernels of lbm_pop.hip, lbm_home.hip,
+// fields.hip and geom.hip on the solver's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/fluidsim.h"
+#include "fs_internal.hpp"
+
+namespace fs {
+bool read_stl(const char* path, std::vector<float>& xyz);
+bool vtk_scalar(const char* path, const float* h, int nx, int ny, int nz);
+bool vtk_vector(const char* path, const float* a, const float* b, const float* c, int nx, int ny,
+                int nz);
+}  // namespace fs
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define FS_HIP(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess) return fail(FS_ERR_DEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+// Runs `body` and maps escaping C++ exceptions to status codes.
+template <class Fn>
+int guarded(Fn&& body) {
+  try {
+    return body();
+  } catch (const std::bad_alloc&) {
+    return fail(FS_ERR_ALLOC, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(FS_ERR_INVALID, "%s", e.what());
+  } catch (...) {
+    return fail(FS_ERR_INVALID, "unknown C++ exception");
+  }
+}
+
+bool fault_injected(const char* what) {
+  const char* v = std::getenv("FS_FAULT_INJECT");
+  return v && std::strstr(v, what) != nullptr;
+}
+
+}  // namespace
+
+struct fs_solver {
+  int method = 0;
+  fs::Geo g{};
+  fs::Phys ph{};
+  float tau = 0.6f;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;
+  // populations (BGK / Hybrid)
+  float* F = nullptr;
+  float* macro = nullptr;  // 4 planes: rho, ux, uy, uz
+  int parity = 0;          // AA representation: 0 natural, 1 swapped
+  // moments (HOME)
+  float* mbuf[2] = {nullptr, nullptr};
+  int cur = 0;
+  // flags
+  uint8_t* flags = nullptr;  // derived flag byte used by the kernels
+  uint8_t* stage = nullptr;  // public flag bytes uploaded from the host
+  std::vector<uint8_t> hflags;
+  float* inlet = nullptr;  // ny*nz
+  float* scratch = nullptr;  // 19 planes for state conversion (lazy)
+
+  float* mom_cur() const { return mbuf[cur]; }
+};
+
+namespace {
+
+int check_solver(const fs_solver* s) {
+  if (!s) return fail(FS_ERR_INVALID, "null solver handle");
+  return FS_OK;
+}
+
+void free_solver(fs_solver* s) {
+  if (!s) return;
+  if (s->own) (void)hipStreamSynchronize(s->own);
+  (void)hipFree(s->F);
+  (void)hipFree(s->macro);
+  (void)hipFree(s->mbuf[0]);
+  (void)hipFree(s->mbuf[1]);
+  (void)hipFree(s->flags);
+  (void)hipFree(s->stage);
+  (void)hipFree(s->inlet);
+  (void)hipFree(s->scratch);
+  if (s->own) (void)hipStreamDestroy(s->own);
+  delete s;
+}
+
+fs::PopLaunch pop_args(fs_solver* s) {
+  fs::PopLaunch a;
+  a.F = s->F;
+  a.flags = s->flags;
+  a.rho = s->macro;
+  a.ux = s->macro + s->g.P;
+  a.uy = s->macro + 2 * s->g.P;
+  a.uz = s->macro + 3 * s->g.P;
+  a.inlet = s->inlet;
+  return a;
+}
+
+int upload_flags(fs_solver* s) {
+  FS_HIP(hipMemcpyAsync(s->stage, s->hflags.data(), s->g.n, hipMemcpyHostToDevice, s->stream));
+  FS_HIP(fs::launch_derive_flags(s->g, s->stage, s->flags, s->stream));
+  FS_HIP(hipStreamSynchronize(s->stream));
+  return FS_OK;
+}
+
+int validate_flags(const fs::Geo& g, const uint8_t* fl) {
+  for (int z = 0; z < g.nz; ++z)
+    for (int y = 0; y < g.ny; ++y) {
+      const size_t row = (size_t(z) * g.ny + y) * g.nx;
+      for (int x = 0; x < g.nx; ++x) {
+        const uint8_t f = fl[row + x];
+        if ((f & FS_FLAG_INLET) && (f & (FS_FLAG_SOLID | FS_FLAG_OUTLET)))
+          return fail(FS_ERR_INVALID, "inlet cell (%d,%d,%d) is also solid or outlet", x, y, z);
+        if (f & FS_FLAG_OUTLET) {
+          if (f & FS_FLAG_SOLID)
+            return fail(FS_ERR_INVALID, "outlet cell (%d,%d,%d) is solid", x, y, z);
+          if (x < 1) return fail(FS_ERR_INVALID, "outlet cell (%d,%d,%d) at x=0", x, y, z);
+          if (fl[row + x - 1] & (FS_FLAG_SOLID | FS_FLAG_INLET | FS_FLAG_OUTLET))
+            return fail(FS_ERR_INVALID,
+                        "outlet cell (%d,%d,%d) needs a fluid, non-inlet, non-outlet cell at x-1",
+                        x, y, z);
+        }
+      }
+    }
+  return FS_OK;
+}
+
+int reset_impl(fs_solver* s) {
+  if (s->method == fs::kMethodHOME) {
+    FS_HIP(fs::launch_home_fill(s->g, s->mbuf[0], s->mbuf[1], nullptr, nullptr, nullptr, nullptr,
+                                s->stream));
+    s->cur = 0;
+  } else {
+    FS_HIP(fs::launch_pop_reset(s->g, pop_args(s), s->stream));
+    s->parity = 0;
+  }
+  FS_HIP(hipStreamSynchronize(s->stream));
+  return FS_OK;
+}
+
+int enqueue_steps(fs_solver* s, int nsteps) {
+  if (nsteps < 0) return fail(FS_ERR_INVALID, "nsteps must be >= 0");
+  if (s->method == fs::kMethodHOME) {
+    for (int i = 0; i < nsteps; ++i) {
+      fs::HomeLaunch a{s->mbuf[s->cur], s->mbuf[1 - s->cur], s->flags, s->inlet};
+      FS_HIP(fs::launch_home_step(s->g, s->ph, a, s->stream));
+      s->cur = 1 - s->cur;
+    }
+  } else {
+    const bool hyb = s->method == fs::kMethodHybrid;
+    const fs::PopLaunch a = pop_args(s);
+    for (int i = 0; i < nsteps; ++i) {
+      FS_HIP(fs::launch_pop_step(s->g, s->ph, a, s->parity != 0, hyb, i == nsteps - 1, s->stream));
+      s->parity ^= 1;
+    }
+  }
+  return FS_OK;
+}
+
+float* macro_plane(fs_solver* s, int which) {
+  if (s->method == fs::kMethodHOME) return s->mom_cur() + size_t(which) * s->g.P;
+  return s->macro + size_t(which) * s->g.P;
+}
+
+int ensure_scratch(fs_solver* s) {
+  if (s->scratch) return FS_OK;
+  FS_HIP(hipMalloc((void**)&s->scratch, sizeof(float) * 19 * s->g.P));
+  return FS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fs_version(void) { return FS_ABI_VERSION; }
+
+const char* fs_last_error(void) { return g_err.c_str(); }
+
+int fs_device_count(int* count) {
+  if (!count) return fail(FS_ERR_INVALID, "null count");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  *count = n;
+  return FS_OK;
+}
+
+int fs_solver_create(int method, int nx, int ny, int nz, float tau, float fx, float fy, float fz,
+                     fs_solver** out) {
+  return guarded([&]() -> int {
+    if (!out) return fail(FS_ERR_INVALID, "null output handle");
+    *out = nullptr;
+    if (method < 0 || method > 2) return fail(FS_ERR_INVALID, "unknown method %d", method);
+    if (nx < 2 || ny < 2 || nz < 1)
+      return fail(FS_ERR_INVALID, "grid %dx%dx%d too small (nx,ny >= 2, nz >= 1)", nx, ny, nz);
+    const unsigned long long n = (unsigned long long)nx * ny * nz;
+    if (n >= (1ull << 31)) return fail(FS_ERR_INVALID, "grid of %llu cells exceeds 2^31", n);
+    if (!(tau > 0.5f) || !std::isfinite(tau))
+      return fail(FS_ERR_INVALID, "tau must be finite and > 0.5");
+    if (fault_injected("alloc")) return fail(FS_ERR_ALLOC, "injected allocation failure");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+      return fail(FS_ERR_DEVICE, "no HIP device available");
+    fs_solver* s = new fs_solver();
+    s->method = method;
+    s->g = fs::make_geo(nx, ny, nz);
+    s->tau = tau;
+    const float omega = 1.0f / tau;
+    s->ph = {omega, omega, omega, fx, fy, fz};
+    s->hflags.assign(s->g.n, 0);
+    auto bail = [&](int code) {
+      free_solver(s);
+      return code;
+    };
+    const size_t P = s->g.P;
+    if (hipStreamCreate(&s->own) != hipSuccess) return bail(fail(FS_ERR_DEVICE, "stream"));
+    s->stream = s->own;
+    hipError_t e = hipSuccess;
+    if (method == fs::kMethodHOME) {
+      e = hipMalloc((void**)&s->mbuf[0], sizeof(float) * 10 * P);
+      if (e == hipSuccess) e = hipMalloc((void**)&s->mbuf[1], sizeof(float) * 10 * P);
+    } else {
+      e = hipMalloc((void**)&s->F, sizeof(float) * 19 * P);
+      if (e == hipSuccess) e = hipMalloc((void**)&s->macro, sizeof(float) * 4 * P);
+    }
+    if (e == hipSuccess) e = hipMalloc((void**)&s->flags, P);
+    if (e == hipSuccess) e = hipMalloc((void**)&s->stage, P);
+    if (e == hipSuccess) e = hipMalloc((void**)&s->inlet, sizeof(float) * size_t(ny) * nz);
+    if (e != hipSuccess)
+      return bail(fail(FS_ERR_ALLOC, "device allocation failed: %s", hipGetErrorString(e)));
+    if (hipMemsetAsync(s->inlet, 0, sizeof(float) * size_t(ny) * nz, s->stream) != hipSuccess)
+      return bail(fail(FS_ERR_DEVICE, "memset"));
+    int rc = upload_flags(s);
+    if (rc == FS_OK) rc = reset_impl(s);
+    if (rc != FS_OK) return bail(rc);
+    *out = s;
+    return FS_OK;
+  });
+}
+
+int fs_solver_destroy(fs_solver* s) {
+  if (!s) return FS_OK;
+  free_solver(s);
+  return FS_OK;
+}
+
+int fs_solver_dims(const fs_solver* s, int* nx, int* ny, int* nz, size_t* n) {
+  if (int rc = check_solver(s)) return rc;
+  if (nx) *nx = s->g.nx;
+  if (ny) *ny = s->g.ny;
+  if (nz) *nz = s->g.nz;
+  if (n) *n = s->g.n;
+  return FS_OK;
+}
+
+int fs_solver_method(const fs_solver* s, int* method) {
+  if (int rc = check_solver(s)) return rc;
+  if (!method) return fail(FS_ERR_INVALID, "null method");
+  *method = s->method;
+  return FS_OK;
+}
+
+int fs_solver_set_solid_mask(fs_solver* s, const uint8_t* mask) {
+  if (int rc = check_solver(s)) return rc;
+  if (!mask) return fail(FS_ERR_INVALID, "null mask");
+  return guarded([&]() -> int {
+    for (size_t i = 0; i < s->g.n; ++i)
+      s->hflags[i] = uint8_t((mask[i] ? FS_FLAG_SOLID : 0) | (s->hflags[i] & FS_FLAG_BAND));
+    return upload_flags(s);
+  });
+}
+
+int fs_solver_set_legacy_mapping(fs_solver* s, const uint8_t* is_legacy, const int*) {
+  if (int rc = check_solver(s)) return rc;
+  if (s->method != fs::kMethodHybrid)
+    return fail(FS_ERR_STATE, "setLegacyMapping applies to the Hybrid method only");
+  return guarded([&]() -> int {
+    for (size_t i = 0; i < s->g.n; ++i) {
+      const bool b = is_legacy && is_legacy[i];
+      s->hflags[i] = uint8_t((s->hflags[i] & ~FS_FLAG_BAND) | (b ? FS_FLAG_BAND : 0));
+    }
+    return upload_flags(s);
+  });
+}
+
+int fs_solver_set_flags(fs_solver* s, const uint8_t* fl) {
+  if (int rc = check_solver(s)) return rc;
+  if (!fl) return fail(FS_ERR_INVALID, "null flags");
+  return guarded([&]() -> int {
+    if (int rc = validate_flags(s->g, fl)) return rc;
+    for (size_t i = 0; i < s->g.n; ++i) s->hflags[i] = fl[i] & fs::F_PUBLIC;
+    return upload_flags(s);
+  });
+}
+
+int fs_solver_copy_flags(fs_solver* s, uint8_t* out) {
+  if (int rc = check_solver(s)) return rc;
+  if (!out) return fail(FS_ERR_INVALID, "null output");
+  std::memcpy(out, s->hflags.data(), s->g.n);
+  return FS_OK;
+}
+
+int fs_solver_set_inlet_profile(fs_solver* s, const float* ux) {
+  if (int rc = check_solver(s)) return rc;
+  const size_t bytes = sizeof(float) * size_t(s->g.ny) * s->g.nz;
+  if (ux)
+    FS_HIP(hipMemcpyAsync(s->inlet, ux, bytes, hipMemcpyHostToDevice, s->stream));
+  else
+    FS_HIP(hipMemsetAsync(s->inlet, 0, bytes, s->stream));
+  FS_HIP(hipStreamSynchronize(s->stream));
+  return FS_OK;
+}
+
+int fs_solver_set_force(fs_solver* s, float fx, float fy, float fz) {
+  if (int rc = check_solver(s)) return rc;
+  s->ph.fx = fx;
+  s->ph.fy = fy;
+  s->ph.fz = fz;
+  return FS_OK;
+}
+
+int fs_solver_set_omegas(fs_solver* s, float oL, float oH, float oS) {
+  if (int rc = check_solver(s)) return rc;
+  if (s->method != fs::kMethodHybrid)
+    return fail(FS_ERR_STATE, "relaxation rates apply to the Hybrid method only");
+  s->ph.omegaL = oL;
+  s->ph.omegaH = oH;
+  s->ph.omegaS = oS;
+  return FS_OK;
+}
+
+int fs_solver_reset(fs_solver* s) {
+  if (int rc = check_solver(s)) return rc;
+  return reset_impl(s);
+}
+
+int fs_solver_reinit_from_macro(fs_solver* s, const float* rho, const float* ux, const float* uy,
+                                const float* uz) {
+  if (int rc = check_solver(s)) return rc;
+  if (s->method == fs::kMethodHOME) {
+    // kern_reinit_from_macro (lbm3d_home.cu:109-137) into both buffers
+    FS_HIP(fs::launch_home_fill(s->g, s->mbuf[0], s->mbuf[1], rho, ux, uy, uz, s->stream));
+    s->cur = 0;
+  } else if (s->method == fs::kMethodBGK) {
+    // a null rho means reset() (lbm3d_legacy.cu:174-178)
+    if (!rho) return reset_impl(s);
+    FS_HIP(fs::launch_pop_reinit(s->g, pop_args(s), 0, rho, ux, uy, uz, s->stream));
+    s->parity = 0;
+  } else {
+    FS_HIP(fs::launch_pop_reinit(s->g, pop_args(s), 1, rho, ux, uy, uz, s->stream));
+    s->parity = 0;
+  }
+  FS_HIP(hipStreamSynchronize(s->stream));
+  return FS_OK;
+}
+
+int fs_solver_step(fs_solver* s, int nsteps) {
+  if (int rc = check_solver(s)) return rc;
+  if (int rc = enqueue_steps(s, nsteps)) return rc;
+  FS_HIP(hipStreamSynchronize(s->stream));
+  return FS_OK;
+}
+
+int fs_solver_step_async(fs_solver* s, int nsteps) {
+  if (int rc = check_solver(s)) return rc;
+  return enqueue_steps(s, nsteps);
+}
+
+int fs_solver_set_stream(fs_solver* s, void* st) {
+  if (int rc = check_solver(s)) return rc;
+  FS_HIP(hipStreamSynchronize(s->stream));
+  s->stream = st ? reinterpret_cast<hipStream_t>(st) : s->own;
+  return FS_OK;
+}
+
+int fs_solver_get_stream(const fs_solver* s, void** st) {
+  if (int rc = check_solver(s)) return rc;
+  if (!st) return fail(FS_ERR_INVALID, "null output");
+  *st = reinterpret_cast<void*>(s->stream);
+  return FS_OK;
+}
+
+int fs_solver_synchronize(fs_solver* s) {
+  if (int rc = check_solver(s)) return rc;
+  FS_HIP(hipStreamSynchronize(s->stream));
+  return FS_OK;
+}
+
+int fs_solver_timed_steps(fs_solver* s, int nsteps, float* elapsed_ms) {
+  if (int rc = check_solver(s)) return rc;
+  if (!elapsed_ms) return fail(FS_ERR_INVALID, "null output");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  FS_HIP(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(FS_ERR_DEVICE, "hipEventCreate");
+  }
+  int rc = FS_OK;
+  hipError_t e = hipEventRecord(e0, s->stream);
+  if (e == hipSuccess) rc = enqueue_steps(s, nsteps);
+  if (e == hipSuccess && rc == FS_OK) e = hipEventRecord(e1, s->stream);
+  if (e == hipSuccess && rc == FS_OK) e = hipEventSynchronize(e1);
+  if (e == hipSuccess && rc == FS_OK) e = hipEventElapsedTime(elapsed_ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != FS_OK) return rc;
+  if (e != hipSuccess) return fail(FS_ERR_DEVICE, "timed steps: %s", hipGetErrorString(e));
+  return FS_OK;
+}
+
+int fs_solver_macro_ptr(fs_solver* s, int which, float** d_ptr) {
+  if (int rc = check_solver(s)) return rc;
+  if (which < 0 || which > 3 || !d_ptr) return fail(FS_ERR_INVALID, "bad macro selector");
+  *d_ptr = macro_plane(s, which);
+  return FS_OK;
+}
+
+int fs_solver_copy_macro(fs_solver* s, int which, float* host) {
+  if (int rc = check_solver(s)) return rc;
+  if (which < 0 || which > 3 || !host) return fail(FS_ERR_INVALID, "bad macro selector");
+  FS_HIP(hipMemcpyAsync(host, macro_plane(s, which), sizeof(float) * s->g.n,
+                        hipMemcpyDeviceToHost, s->stream));
+  FS_HIP(hipStreamSynchronize(s->stream));
+  return FS_OK;
+}
+
+int fs_solver_state_size(const fs_solver* s, size_t* nfloats) {
+  if (int rc = check_solver(s)) return rc;
+  if (!nfloats) return fail(FS_ERR_INVALID, "null output");
+  *nfloats = size_t(s->method == fs::kMethodHOME ? 10 : 19) * s->g.n;
+  return FS_OK;
+}
+
+int fs_solver_copy_state(fs_solver* s, float* host) {
+  if (int rc = check_solver(s)) return rc;
+  if (!host) return fail(FS_ERR_INVALID, "null output");
+  const size_t N = s->g.n, P = s->g.P;
+  if (s->method == fs::kMethodHOME) {
+    for (int k = 0; k < 10; ++k)
+      FS_HIP(hipMemcpyAsync(host + size_t(k) * N, s->mom_cur() + size_t(k) * P,
+                            sizeof(float) * N, hipMemcpyDeviceToHost, s->stream));
+  } else {
+    if (int rc = ensure_scratch(s)) return rc;
+    FS_HIP(fs::launch_pop_canonical(s->g, s->F, s->flags, s->parity != 0, s->scratch, s->stream));
+    for (int q = 0; q < 19; ++q)
+      FS_HIP(hipMemcpyAsync(host + size_t(q) * N, s->scratch + size_t(q) * P, sizeof(float) * N,
+                            hipMemcpyDeviceToHost, s->stream));
+  }
+  FS_HIP(hipStreamSynchronize(s->stream));
+  return FS_OK;
+}
+
+int fs_solver_load_state(fs_solver* s, const float* host) {
+  if (int rc = check_solver(s)) return rc;
+  if (!host) return fail(FS_ERR_INVALID, "null input");
+  const size_t N = s->g.n, P = s->g.P;
+  if (s->method == fs::kMethodHOME) {
+    for (int k = 0; k < 10; ++k)
+      FS_HIP(hipMemcpyAsync(s->mom_cur() + size_t(k) * P, host + size_t(k) * N,
+                            sizeof(float) * N, hipMemcpyHostToDevice, s->stream));
+  } else {
+    for (int q = 0; q < 19; ++q)
+      FS_HIP(hipMemcpyAsync(s->F + size_t(q) * P, host + size_t(q) * N, sizeof(float) * N,
+                            hipMemcpyHostToDevice, s->stream));
+    s->parity = 0;
+  }
+  FS_HIP(hipStreamSynchronize(s->stream));
+  return FS_OK;
+}
+
+// ------------------------------------------------------------------ geometry
+
+int fs_load_stl(const char* path, float* tri_xyz, int capacity, int* ntri, float* bbmin3,
+                float* bbmax3) {
+  if (!path) return fail(FS_ERR_INVALID, "null path");
+  return guarded([&]() -> int {
+    std::vector<float> xyz;
+    if (!fs::read_stl(path, xyz)) return fail(FS_ERR_IO, "cannot read STL file %s", path);
+    const int nt = int(xyz.size() / 9);
+    if (ntri) *ntri = nt;
+    float mn[3], mx[3];
+    for (int k = 0; k < 3; ++k) {
+      mn[k] = std::numeric_limits<float>::max();
+      mx[k] = -std::numeric_limits<float>::max();
+    }
+    for (size_t i = 0; i + 2 < xyz.size(); i += 3)
+      for (int k = 0; k < 3; ++k) {
+        // std::min / std::max argument order of update_bb (stl_loader.cpp:19-22)
+        mn[k] = (xyz[i + k] < mn[k]) ? xyz[i + k] : mn[k];
+        mx[k] = (mx[k] < xyz[i + k]) ? xyz[i + k] : mx[k];
+      }
+    if (bbmin3) std::memcpy(bbmin3, mn, sizeof(mn));
+    if (bbmax3) std::memcpy(bbmax3, mx, sizeof(mx));
+    if (tri_xyz) {
+      if (capacity < nt) return fail(FS_ERR_INVALID, "capacity %d < %d triangles", capacity, nt);
+      std::memcpy(tri_xyz, xyz.data(), xyz.size() * sizeof(float));
+    }
+    return FS_OK;
+  });
+}
+
+int fs_voxelize(const float* tri_xyz, int ntri, const float* bbmin3, const float* bbmax3, int nx,
+                int ny, int nz, float uniform_scale, float tx, float ty, float tz, int refine,
+                float threshold, uint8_t* mask_out) {
+  if (nx < 1 || ny < 1 || nz < 1 || !mask_out || !bbmin3 || !bbmax3 || ntri < 0 ||
+      (ntri > 0 && !tri_xyz))
+    return fail(FS_ERR_INVALID, "bad voxelize arguments");
+  return guarded([&]() -> int {
+    const int r = refine < 1 ? 1 : refine;
+    const size_t nf = size_t(nx) * r * size_t(ny) * r * size_t(nz) * r;
+    if (size_t(nx) * r >= (1u << 31) || nf >= (size_t(1) << 40))
+      return fail(FS_ERR_INVALID, "refined grid too large");
+    // compute_fit_transform + make_transformed_tris (voxelize.cpp:83-113)
+    float bb[3];
+    for (int k = 0; k < 3; ++k) bb[k] = bbmax3[k] - bbmin3[k];
+    const float maxDim = std::max(bb[0], std::max(bb[1], bb[2]));
+    const float sc = (maxDim > 0.0f) ? (uniform_scale / maxDim) : 1.0f;
+    const float tr[3] = {tx, ty, tz};
+    float trans[3];
+    for (int k = 0; k < 3; ++k) {
+      const float center = 0.5f * (bbmin3[k] + bbmax3[k]);
+      trans[k] = tr[k] - sc * center;
+    }
+    std::vector<fs::TriF> tris(static_cast<size_t>(ntri));
+    for (int t = 0; t < ntri; ++t)
+      for (int j = 0; j < 9; ++j) tris[t].v[j] = sc * tri_xyz[size_t(t) * 9 + j] + trans[j % 3];
+    fs::TriF* d_tris = nullptr;
+    uint8_t* d_fine = nullptr;
+    uint8_t* d_out = nullptr;
+    auto cleanup = [&]() {
+      (void)hipFree(d_tris);
+      (void)hipFree(d_fine);
+      (void)hipFree(d_out);
+    };
+    hipStream_t st = nullptr;
+    hipError_t e = hipSuccess;
+    if (ntri > 0) e = hipMalloc((void**)&d_tris, sizeof(fs::TriF) * ntri);
+    if (e == hipSuccess) e = hipMalloc((void**)&d_fine, nf);
+    if (e == hipSuccess && r > 1) e = hipMalloc((void**)&d_out, size_t(nx) * ny * nz);
+    if (e != hipSuccess) {
+      cleanup();
+      return fail(FS_ERR_ALLOC, "device allocation failed: %s", hipGetErrorString(e));
+    }
+    if (ntri > 0)
+      e = hipMemcpy(d_tris, tris.data(), sizeof(fs::TriF) * ntri, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = fs::voxelize_device(d_tris, ntri, nx * r, ny * r, nz * r, d_fine, st);
+    if (e == hipSuccess && r > 1)
+      e = fs::downsample_device(d_fine, r, nx, ny, nz, threshold, d_out, st);
+    if (e == hipSuccess)
+      e = hipMemcpy(mask_out, r > 1 ? d_out : d_fine, size_t(nx) * ny * nz,
+                    hipMemcpyDeviceToHost);
+    cleanup();
+    if (e != hipSuccess) return fail(FS_ERR_DEVICE, "voxelize: %s", hipGetErrorString(e));
+    return FS_OK;
+  });
+}
+
+int fs_band(const uint8_t* solid, int nx, int ny, int nz, int d0, uint8_t* band_out) {
+  if (!solid || !band_out || nx < 1 || ny < 1 || nz < 1)
+    return fail(FS_ERR_INVALID, "bad band arguments");
+  const size_t n = size_t(nx) * ny * nz;
+  uint8_t* d_solid = nullptr;
+  uint8_t* d_band = nullptr;
+  uint8_t* d_scr = nullptr;
+  hipError_t e = hipMalloc((void**)&d_solid, n);
+  if (e == hipSuccess) e = hipMalloc((void**)&d_band, n);
+  if (e == hipSuccess) e = hipMalloc((void**)&d_scr, n * sizeof(int));
+  if (e == hipSuccess) e = hipMemcpy(d_solid, solid, n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = fs::band_device(d_solid, nx, ny, nz, d0 < 0 ? 0 : d0, d_band, d_scr, nullptr);
+  if (e == hipSuccess) e = hipMemcpy(band_out, d_band, n, hipMemcpyDeviceToHost);
+  (void)hipFree(d_solid);
+  (void)hipFree(d_band);
+  (void)hipFree(d_scr);
+  if (e != hipSuccess) return fail(FS_ERR_DEVICE, "band: %s", hipGetErrorString(e));
+  return FS_OK;
+}
+
+// ------------------------------------------------------------------ fields
+
+int fs_speed(const float* u, const float* v, const float* w, float* out, size_t n) {
+  if (n && (!u || !v || !w || !out)) return fail(FS_ERR_INVALID, "null field");
+  FS_HIP(fs::launch_speed_k(u, v, w, out, n, nullptr));
+  return FS_OK;
+}
+
+int fs_rho_gauss(float* rho, int nx, int ny, int nz, float cx, float cy, float cz, float sigma,
+                 float amp) {
+  if (!rho || nx < 1 || ny < 1 || nz < 1) return fail(FS_ERR_INVALID, "bad gauss arguments");
+  FS_HIP(fs::launch_rho_gauss(rho, nx, ny, nz, cx, cy, cz, sigma, amp, nullptr));
+  return FS_OK;
+}
+
+int fs_rho_slab(float* rho, int nx, int ny, int nz, int axis, float amp, float center,
+                float width) {
+  if (!rho || nx < 1 || ny < 1 || nz < 1) return fail(FS_ERR_INVALID, "bad slab arguments");
+  FS_HIP(fs::launch_rho_slab(rho, nx, ny, nz, axis, amp, center, width, nullptr));
+  return FS_OK;
+}
+
+int fs_reinit_equilibrium(const float* rho, const float* ux, const float* uy, const float* uz,
+                          float* f, size_t n) {
+  if (n && (!rho || !f)) return fail(FS_ERR_INVALID, "null rho or f");
+  FS_HIP(fs::launch_reinit_cs2(rho, ux, uy, uz, f, n, n, nullptr));
+  return FS_OK;
+}
+
+int fs_write_scalar_vtk(const char* path, const float* d_field, int nx, int ny, int nz) {
+  if (!path || !d_field || nx < 1 || ny < 1 || nz < 1)
+    return fail(FS_ERR_INVALID, "bad VTK arguments");
+  return guarded([&]() -> int {
+    const size_t n = size_t(nx) * ny * nz;
+    std::vector<float> h(n);
+    FS_HIP(hipMemcpy(h.data(), d_field, n * sizeof(float), hipMemcpyDeviceToHost));
+    if (!fs::vtk_scalar(path, h.data(), nx, ny, nz))
+      return fail(FS_ERR_IO, "failed to open VTK file %s", path);
+    return FS_OK;
+  });
+}
+
+int fs_write_vector_vtk(const char* path, const float* a, const float* b, const float* c, int nx,
+                        int ny, int nz) {
+  if (!path || !a || !b || !c || nx < 1 || ny < 1 || nz < 1)
+    return fail(FS_ERR_INVALID, "bad VTK arguments");
+  return guarded([&]() -> int {
+    const size_t n = size_t(nx) * ny * nz;
+    std::vector<float> h(3 * n);
+    FS_HIP(hipMemcpy(h.data(), a, n * sizeof(float), hipMemcpyDeviceToHost));
+    FS_HIP(hipMemcpy(h.data() + n, b, n * sizeof(float), hipMemcpyDeviceToHost));
+    FS_HIP(hipMemcpy(h.data() + 2 * n, c, n * sizeof(float), hipMemcpyDeviceToHost));
+    if (!fs::vtk_vector(path, h.data(), h.data() + n, h.data() + 2 * n, nx, ny, nz))
+      return fail(FS_ERR_IO, "failed to open VTK file %s", path);
+    return FS_OK;
+  });
+}
+
+int fs_device_malloc(void** p, size_t bytes) {
+  if (!p) return fail(FS_ERR_INVALID, "null output");
+  *p = nullptr;
+  if (fault_injected("alloc")) return fail(FS_ERR_ALLOC, "injected allocation failure");
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) return fail(FS_ERR_ALLOC, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  return FS_OK;
+}
+int fs_device_free(void* p) {
+  FS_HIP(hipFree(p));
+  return FS_OK;
+}
+int fs_memcpy_htod(void* d, const void* h, size_t bytes) {
+  FS_HIP(hipMemcpy(d, h, bytes, hipMemcpyHostToDevice));
+  return FS_OK;
+}
+int fs_memcpy_dtoh(void* h, const void* d, size_t bytes) {
+  FS_HIP(hipMemcpy(h, d, bytes, hipMemcpyDeviceToHost));
+  return FS_OK;
+}
+int fs_memset_device(void* d, int value, size_t bytes) {
+  FS_HIP(hipMemset(d, value, bytes));
+  return FS_OK;
+}
+int fs_device_synchronize(void) {
+  FS_HIP(hipDeviceSynchronize());
+  return FS_OK;
+}
+
+int fs_copy_probe(void* dst, const void* src, size_t bytes, int iters, float* elapsed_ms) {
+  if (!dst || !src || !elapsed_ms || iters < 1 || (bytes % 16) != 0 ||
+      (reinterpret_cast<uintptr_t>(dst) % 16) != 0 || (reinterpret_cast<uintptr_t>(src) % 16) != 0)
+    return fail(FS_ERR_INVALID, "bad copy probe arguments");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  FS_HIP(hipEventCreate(&e0));
+  FS_HIP(hipEventCreate(&e1));
+  hipError_t e = hipEventRecord(e0, nullptr);
+  for (int i = 0; i < iters && e == hipSuccess; ++i) e = fs::launch_copy_probe(dst, src, bytes, nullptr);
+  if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  if (e == hipSuccess) e = hipEventElapsedTime(elapsed_ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (e != hipSuccess) return fail(FS_ERR_DEVICE, "copy probe: %s", hipGetErrorString(e));
+  return FS_OK;
+}
+
+// ------------------------------------------------- reference-named helpers
+
+static void die_on(int rc, const char* what) {
+  if (rc == FS_OK) return;
+  std::fprintf(stderr, "[error] %s: %s\n", what, g_err.c_str());
+  std::abort();
+}
+
+void write_scalar_vtk(const char* filename, const float* d_field, int Nx, int Ny, int Nz) {
+  die_on(fs_write_scalar_vtk(filename, d_field, Nx, Ny, Nz), "write_scalar_vtk");
+}
+void write_vector_vtk(const char* filename, const float* d_fx, const float* d_fy,
+                      const float* d_fz, int Nx, int Ny, int Nz) {
+  die_on(fs_write_vector_vtk(filename, d_fx, d_fy, d_fz, Nx, Ny, Nz), "write_vector_vtk");
+}
+void make_rho_gauss(float* d_rho, int Nx, int Ny, int Nz, float cx, float cy, float cz,
+                    float sigma, float amp) {
+  die_on(fs_rho_gauss(d_rho, Nx, Ny, Nz, cx, cy, cz, sigma, amp), "make_rho_gauss");
+}
+void make_rho_slab(float* d_rho, int Nx, int Ny, int Nz, int axis, float amp, float center,
+                   float width) {
+  die_on(fs_rho_slab(d_rho, Nx, Ny, Nz, axis, amp, center, width), "make_rho_slab");
+}
+void reinit_equilibrium_from_macro(const float* d_rho, const float* d_ux, const float* d_uy,
+                                   const float* d_uz, float* d_f, int N) {
+  die_on(fs_reinit_equilibrium(d_rho, d_ux, d_uy, d_uz, d_f, N < 0 ? 0 : size_t(N)),
+         "reinit_equilibrium_from_macro");
+}
+void launch_speed(const float* u, const float* v, const float* w, float* out, int N) {
+  die_on(fs_speed(u, v, w, out, N < 0 ? 0 : size_t(N)), "launch_speed");
+}
+
+}  // extern "C"

@@ -1,0 +1,169 @@
+// d3q19_dev.hpp -- per-direction D3Q19 arithmetic for the HIP kernels.
+//
+// Each helper is specialised on the direction index at compile time, so the
+// lattice constants fold away.  Terms that the reference multiplies by a zero
+// velocity component are dropped and terms multiplied by +-1 become +-x:
+// both are exact in IEEE arithmetic (x*1 == x, x*-1 == -x, y + (+-0) == y for
+// any nonzero y), so every stored value keeps the reference's bits while
+// the non-zero terms are combined in the reference's order.  The library is
+// built with -ffp-contract=off so no multiply-add is fused.
+#pragma once
+
+#include <type_traits>
+
+#include "fs_internal.hpp"
+
+namespace fs {
+
+template <int Q>
+struct Dir {
+  static constexpr int cx = CX[Q], cy = CY[Q], cz = CZ[Q], opp = OPP[Q];
+  // w19 (lbm3d_legacy.cu:25-28)
+  static constexpr float w = Q == 0 ? 1.0f / 3.0f : (Q <= 6 ? 1.0f / 18.0f : 1.0f / 36.0f);
+};
+
+template <int C>
+__device__ __forceinline__ float sg(float v) {
+  return C > 0 ? v : -v;
+}
+
+// e_q . u = cx*ux + cy*uy + cz*uz with the zero terms dropped
+template <int Q>
+__device__ __forceinline__ float edot(float ux, float uy, float uz) {
+  constexpr int a = Dir<Q>::cx, b = Dir<Q>::cy, c = Dir<Q>::cz;
+  if constexpr (a == 0 && b == 0 && c == 0) {
+    return 0.0f;
+  } else if constexpr (b == 0 && c == 0) {
+    return sg<a>(ux);
+  } else if constexpr (a == 0 && c == 0) {
+    return sg<b>(uy);
+  } else if constexpr (a == 0 && b == 0) {
+    return sg<c>(uz);
+  } else if constexpr (c == 0) {
+    return sg<a>(ux) + sg<b>(uy);
+  } else if constexpr (b == 0) {
+    return sg<a>(ux) + sg<c>(uz);
+  } else {
+    return sg<b>(uy) + sg<c>(uz);
+  }
+}
+
+// acc += v * c for c in {-1,0,1}
+template <int C>
+__device__ __forceinline__ void acc1(float& acc, float v) {
+  if constexpr (C > 0) acc += v;
+  if constexpr (C < 0) acc -= v;
+}
+
+// w_q * rho for the three weight classes
+struct WRho {
+  float w0, w1, w7;
+  __device__ __forceinline__ WRho(float r)
+      : w0(Dir<0>::w * r), w1(Dir<1>::w * r), w7(Dir<7>::w * r) {}
+  template <int Q>
+  __device__ __forceinline__ float get() const {
+    return Q == 0 ? w0 : (Q <= 6 ? w1 : w7);
+  }
+};
+
+// feq (lbm3d_legacy.cu:54-58), feq19 (lbm3d_hybrid.cu:37-41) and the
+// equilibrium part of reconstruct_f_post (lbm3d_home.cu:57-59):
+//   w*rho*(1 + 3eu + 4.5eu*eu - 1.5uu)
+template <int Q>
+__device__ __forceinline__ float feq_q(const WRho& wr, float ux, float uy, float uz, float uu15) {
+  const float eu = edot<Q>(ux, uy, uz);
+  return wr.template get<Q>() * (((1.0f + 3.0f * eu) + (4.5f * eu) * eu) - uu15);
+}
+
+// kern_reinit_eq's cs2 form (lbm_reinit.cu:31-49):
+//   w*r*(1 + cu/cs2 + 0.5*(cu*cu)/(cs2*cs2) - 0.5*uu/cs2)
+template <int Q>
+__device__ __forceinline__ float feq_cs2_q(const WRho& wr, float ux, float uy, float uz,
+                                           float uu) {
+  constexpr float cs2 = 1.0f / 3.0f;
+  constexpr float cs4 = cs2 * cs2;
+  const float cu = edot<Q>(ux, uy, uz);
+  return wr.template get<Q>() * (((1.0f + cu / cs2) + (0.5f * (cu * cu)) / cs4) - (0.5f * uu) / cs2);
+}
+
+// 10 HOME moments of one cell: rho, ux, uy, uz, Sxx, Sxy, Sxz, Syy, Syz, Szz
+struct Mom {
+  float v[10];
+};
+
+// reconstruct_f_post (lbm3d_home.cu:37-81) for direction Q
+template <int Q>
+__device__ __forceinline__ float home_rec(const Mom& m, float oneMinusOmega) {
+  constexpr int a = Dir<Q>::cx, b = Dir<Q>::cy, c = Dir<Q>::cz;
+  constexpr float cs2 = 1.0f / 3.0f;
+  constexpr float Qxx = float(a * a) - cs2;
+  constexpr float Qyy = float(b * b) - cs2;
+  constexpr float Qzz = float(c * c) - cs2;
+  constexpr float w = Dir<Q>::w;
+  constexpr float w45 = 4.5f * w;
+  const float rho = m.v[0], ux = m.v[1], uy = m.v[2], uz = m.v[3];
+  const float eu = edot<Q>(ux, uy, uz);
+  const float uu = ux * ux + uy * uy + uz * uz;
+  const float feq = (w * rho) * (((1.0f + 3.0f * eu) + (4.5f * eu) * eu) - 1.5f * uu);
+  // (Qxx*Sxx + Qyy*Syy + Qzz*Szz) + 2*(Qxy*Sxy + Qxz*Sxz + Qyz*Syz); at most
+  // one off-diagonal Q is nonzero (+-1) for a D3Q19 direction
+  float contraction = (Qxx * m.v[4] + Qyy * m.v[7]) + Qzz * m.v[9];
+  if constexpr (a != 0 && b != 0) contraction += 2.0f * sg<a * b>(m.v[5]);
+  if constexpr (a != 0 && c != 0) contraction += 2.0f * sg<a * c>(m.v[6]);
+  if constexpr (b != 0 && c != 0) contraction += 2.0f * sg<b * c>(m.v[8]);
+  const float fneq = w45 * contraction;
+  return feq + oneMinusOmega * fneq;
+}
+
+// Compile-time loop over the 19 directions.
+template <int Q, int END, class Fn>
+__device__ __forceinline__ void for_q(Fn&& fn) {
+  if constexpr (Q < END) {
+    fn(std::integral_constant<int, Q>{});
+    for_q<Q + 1, END>(fn);
+  }
+}
+
+// Periodic neighbour offsets of cell (x,y,z) in the linear index.
+struct Nbr {
+  int xm, xp, ym, yp, zm, zp;
+};
+__device__ __forceinline__ Nbr nbr_offsets(int x, int y, int z, const Geo& g) {
+  Nbr o;
+  o.xm = (x == 0) ? (g.nx - 1) : -1;
+  o.xp = (x == g.nx - 1) ? -(g.nx - 1) : 1;
+  o.ym = (y == 0) ? (g.ny - 1) * g.nx : -g.nx;
+  o.yp = (y == g.ny - 1) ? -(g.ny - 1) * g.nx : g.nx;
+  o.zm = (z == 0) ? int((g.nz - 1) * g.nxy) : -int(g.nxy);
+  o.zp = (z == g.nz - 1) ? -int((g.nz - 1) * g.nxy) : int(g.nxy);
+  return o;
+}
+template <int DX, int DY, int DZ>
+__device__ __forceinline__ int nb_off(const Nbr& o) {
+  int s = 0;
+  if constexpr (DX < 0) s += o.xm;
+  if constexpr (DX > 0) s += o.xp;
+  if constexpr (DY < 0) s += o.ym;
+  if constexpr (DY > 0) s += o.yp;
+  if constexpr (DZ < 0) s += o.zm;
+  if constexpr (DZ > 0) s += o.zp;
+  return s;
+}
+
+// Cell coordinates for the (plane-linear, z) launch grid.
+struct Cell {
+  uint32_t c;
+  int x, y, z;
+};
+__device__ __forceinline__ bool cell_of_thread(const Geo& g, Cell& cl) {
+  const uint32_t cp = blockIdx.x * blockDim.x + threadIdx.x;
+  if (cp >= g.nxy) return false;
+  cl.z = blockIdx.y;
+  const uint32_t y = g.divx.div(cp);
+  cl.y = int(y);
+  cl.x = int(cp - y * uint32_t(g.nx));
+  cl.c = uint32_t(cl.z) * g.nxy + cp;
+  return true;
+}
+
+}  // namespace fs

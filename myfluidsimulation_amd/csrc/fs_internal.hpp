@@ -1,0 +1,146 @@
+// fs_internal.hpp -- definitions shared by the HIP kernels and the C ABI layer.
+//
+// Data layout in HBM (one solver):
+//   populations  F[q*P + c]   19 planes, one buffer, updated in place (AA)
+//   moments      m[k*P + c]   10 planes, two buffers (HOME ping-pong)
+//   macros       rho/ux/uy/uz separate planes of P floats (BGK / Hybrid)
+//   flags        uint8 per cell
+// c = (z*ny + y)*nx + x (x fastest, as lbm3d_legacy.cu:33-35); P is the cell
+// count rounded up to 64 floats so every plane starts on a 256-byte boundary.
+// Offsets are 64-bit: 19*P exceeds 2^31 at 512^3.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace fs {
+
+// D3Q19 velocity set in the reference order (lbm3d_legacy.cu:16-32).
+constexpr int CX[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+constexpr int CY[19] = {0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1};
+constexpr int CZ[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1};
+constexpr int OPP[19] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
+
+// Flag byte: the low four bits are the public FS_FLAG_* bits; the upper bits
+// are derived on the device after every flag upload.
+constexpr uint8_t F_SOLID = 0x01;
+constexpr uint8_t F_BAND = 0x02;
+constexpr uint8_t F_INLET = 0x04;
+constexpr uint8_t F_OUTLET = 0x08;
+constexpr uint8_t F_NEAR = 0x10;    // one of the 18 periodic neighbours is solid
+constexpr uint8_t F_FEEDER = 0x20;  // cell x+1 is an outlet: emit for it too
+constexpr uint8_t F_PUBLIC = 0x0F;
+
+constexpr int kMethodBGK = 0;
+constexpr int kMethodHOME = 1;
+constexpr int kMethodHybrid = 2;
+
+inline size_t padded_cells(size_t n) { return (n + 63) & ~size_t(63); }
+
+// Unsigned 32-bit division by a runtime-invariant divisor with a multiply
+// and shifts (Granlund & Montgomery, "Division by invariant integers using
+// multiplication", PLDI 1994, round-up variant with the add fix-up).
+struct FastDiv {
+  uint32_t d, m, s1, s2;
+  static FastDiv make(uint32_t d) {
+    FastDiv f;
+    f.d = d;
+    uint32_t l = 0;
+    while ((uint64_t(1) << l) < d) ++l;
+    f.m = uint32_t(((uint64_t(1) << 32) * ((uint64_t(1) << l) - d)) / d + 1);
+    f.s1 = l < 1 ? l : 1;
+    f.s2 = l > 0 ? l - 1 : 0;
+    return f;
+  }
+  __host__ __device__ inline uint32_t div(uint32_t n) const {
+#ifdef __HIP_DEVICE_COMPILE__
+    uint32_t t = __umulhi(m, n);
+#else
+    uint32_t t = uint32_t((uint64_t(m) * n) >> 32);
+#endif
+    return (t + ((n - t) >> s1)) >> s2;
+  }
+};
+
+// Grid geometry passed by value to every lattice kernel.
+struct Geo {
+  int nx, ny, nz;
+  uint32_t nxy;   // nx*ny
+  uint32_t n;     // cells
+  size_t P;       // plane stride in floats
+  FastDiv divx;   // c_plane / nx
+};
+
+inline Geo make_geo(int nx, int ny, int nz) {
+  Geo g;
+  g.nx = nx;
+  g.ny = ny;
+  g.nz = nz;
+  g.nxy = uint32_t(nx) * uint32_t(ny);
+  g.n = g.nxy * uint32_t(nz);
+  g.P = padded_cells(g.n);
+  g.divx = FastDiv::make(uint32_t(nx));
+  return g;
+}
+
+// Relaxation and forcing parameters of one solver.
+struct Phys {
+  float omegaL, omegaH, omegaS;  // BGK / hybrid-bulk rates (all 1/tau by default)
+  float fx, fy, fz;              // velocity shift u_eq = u + F
+};
+
+// ---- launchers (defined in the .hip files; all enqueue on `st`) ----
+struct PopLaunch {
+  float* F;
+  const uint8_t* flags;
+  float *rho, *ux, *uy, *uz;
+  const float* inlet;  // ny*nz inlet x-velocity
+};
+hipError_t launch_pop_step(const Geo& g, const Phys& ph, const PopLaunch& a, bool odd, bool hybrid,
+                           bool write_macro, hipStream_t st);
+hipError_t launch_pop_reset(const Geo& g, const PopLaunch& a, hipStream_t st);
+// mode 0: Legacy cs2 form, macros copied (null u -> 0)
+// mode 1: Hybrid feq19 form, macros copied (null rho -> 1, null u -> 0)
+hipError_t launch_pop_reinit(const Geo& g, const PopLaunch& a, int mode, const float* rho,
+                             const float* ux, const float* uy, const float* uz, hipStream_t st);
+// populations in the swapped (odd) AA representation -> reference order
+hipError_t launch_pop_canonical(const Geo& g, const float* F, const uint8_t* flags, bool swapped,
+                                float* out, hipStream_t st);
+hipError_t launch_derive_flags(const Geo& g, const uint8_t* in, uint8_t* out, hipStream_t st);
+
+struct HomeLaunch {
+  const float* m;
+  float* mn;
+  const uint8_t* flags;
+  const float* inlet;
+};
+hipError_t launch_home_step(const Geo& g, const Phys& ph, const HomeLaunch& a, hipStream_t st);
+hipError_t launch_home_fill(const Geo& g, float* m, float* mn, const float* rho, const float* ux,
+                            const float* uy, const float* uz, hipStream_t st);
+
+// fields
+hipError_t launch_speed_k(const float* u, const float* v, const float* w, float* out, size_t n,
+                          hipStream_t st);
+hipError_t launch_rho_gauss(float* rho, int nx, int ny, int nz, float cx, float cy, float cz,
+                            float sigma, float amp, hipStream_t st);
+hipError_t launch_rho_slab(float* rho, int nx, int ny, int nz, int axis, float amp, float center,
+                           float width, hipStream_t st);
+hipError_t launch_reinit_cs2(const float* rho, const float* ux, const float* uy, const float* uz,
+                             float* f, size_t n, size_t stride, hipStream_t st);
+
+// plain streaming copy (bandwidth ceiling probe)
+hipError_t launch_copy_probe(void* dst, const void* src, size_t bytes, hipStream_t st);
+
+// geometry
+struct TriF {
+  float v[9];
+};
+hipError_t voxelize_device(const TriF* d_tris, int ntri, int nx, int ny, int nz, uint8_t* d_out,
+                           hipStream_t st);
+hipError_t downsample_device(const uint8_t* d_fine, int refine, int nx, int ny, int nz,
+                             float threshold, uint8_t* d_out, hipStream_t st);
+hipError_t band_device(const uint8_t* d_solid, int nx, int ny, int nz, int d0, uint8_t* d_band,
+                       uint8_t* d_scratch, hipStream_t st);
+
+}  // namespace fs

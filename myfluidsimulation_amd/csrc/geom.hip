@@ -1,0 +1,297 @@
+// geom.hip -- STL voxelizer and Hybrid band construction on gfx950.
+//
+// Voxelizer (voxelize.cpp:115-243):
+//   1. surface: one workgroup per triangle walks the triangle's voxel index
+//      box and runs the triangle/box separating-axis test per voxel
+//      (triBoxOverlap_SAT, voxelize.cpp:23-81), same float expressions and
+//      thresholds, built without contraction, so every decision matches the
+//      CPU rule.  Stores of the byte 1 from different triangles may land on
+//      one voxel; they write the same value.
+//   2. exterior: the reference's 6-neighbour BFS from the six faces is a
+//      reachability set, so any order reaches the same fixpoint.  Here it is
+//      computed by repeated line sweeps along x, y and z until a full round
+//      changes nothing.
+//   3. solid = surface or not exterior; optional AMR coverage downsample.
+// Band (sim_runner.hpp:28-91): a level-synchronous BFS, one launch per
+// distance ring, non-periodic, then band = fluid cells with 1 <= dist <= d0.
+#include "fs_internal.hpp"
+
+namespace fs {
+
+namespace {
+
+struct F3 {
+  float x, y, z;
+};
+__device__ __forceinline__ F3 sub(F3 a, F3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ F3 cross(F3 a, F3 b) {
+  // glm::cross(x, y) = (x.y*y.z - y.y*x.z, x.z*y.x - y.z*x.x, x.x*y.y - y.x*x.y)
+  return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
+}
+// std::min / std::max semantics: min(a,b) = (b < a) ? b : a
+__device__ __forceinline__ float smin(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float smax(float a, float b) { return (a < b) ? b : a; }
+
+__device__ bool tri_box(F3 c, F3 half, F3 v0, F3 v1, F3 v2) {
+  const F3 tv0 = sub(v0, c), tv1 = sub(v1, c), tv2 = sub(v2, c);
+  const float mnx = smin(smin(tv0.x, tv1.x), tv2.x), mxx = smax(smax(tv0.x, tv1.x), tv2.x);
+  const float mny = smin(smin(tv0.y, tv1.y), tv2.y), mxy = smax(smax(tv0.y, tv1.y), tv2.y);
+  const float mnz = smin(smin(tv0.z, tv1.z), tv2.z), mxz = smax(smax(tv0.z, tv1.z), tv2.z);
+  if (mxx < -half.x || mnx > half.x) return false;
+  if (mxy < -half.y || mny > half.y) return false;
+  if (mxz < -half.z || mnz > half.z) return false;
+  const F3 e0 = sub(tv1, tv0), e1 = sub(tv2, tv1), e2 = sub(tv0, tv2);
+  const F3 n = cross(e0, e1);
+  const float r = half.x * fabsf(n.x) + half.y * fabsf(n.y) + half.z * fabsf(n.z);
+  const float s = fabsf(n.x * tv0.x + n.y * tv0.y + n.z * tv0.z);
+  if (s > r) return false;
+  const F3 edges[3] = {e0, e1, e2};
+#pragma unroll
+  for (int ei = 0; ei < 3; ++ei) {
+    const F3 e = edges[ei];
+    if (fabsf(e.x) + fabsf(e.y) + fabsf(e.z) < 1e-12f) continue;
+#pragma unroll
+    for (int ai = 0; ai < 3; ++ai) {
+      const F3 ax = {ai == 0 ? 1.0f : 0.0f, ai == 1 ? 1.0f : 0.0f, ai == 2 ? 1.0f : 0.0f};
+      const F3 L = cross(e, ax);
+      const float Llen2 = L.x * L.x + L.y * L.y + L.z * L.z;
+      if (Llen2 < 1e-20f) continue;
+      const float p0 = L.x * tv0.x + L.y * tv0.y + L.z * tv0.z;
+      const float p1 = L.x * tv1.x + L.y * tv1.y + L.z * tv1.z;
+      const float p2 = L.x * tv2.x + L.y * tv2.y + L.z * tv2.z;
+      const float minp = smin(p0, smin(p1, p2));
+      const float maxp = smax(p0, smax(p1, p2));
+      const float rProj = half.x * fabsf(L.x) + half.y * fabsf(L.y) + half.z * fabsf(L.z);
+      if (maxp < -rProj || minp > rProj) return false;
+    }
+  }
+  return true;
+}
+
+__global__ void __launch_bounds__(256) k_surface(const TriF* __restrict__ tris, int nx, int ny,
+                                                 int nz, uint8_t* __restrict__ surf) {
+  const TriF t = tris[blockIdx.x];
+  const F3 v0 = {t.v[0], t.v[1], t.v[2]}, v1 = {t.v[3], t.v[4], t.v[5]},
+           v2 = {t.v[6], t.v[7], t.v[8]};
+  const float eps = 1e-6f;
+  // index box (voxelize.cpp:127-139)
+  const float minx = smin(v0.x, smin(v1.x, v2.x)) - eps;
+  const float miny = smin(v0.y, smin(v1.y, v2.y)) - eps;
+  const float minz = smin(v0.z, smin(v1.z, v2.z)) - eps;
+  const float maxx = smax(v0.x, smax(v1.x, v2.x)) + eps;
+  const float maxy = smax(v0.y, smax(v1.y, v2.y)) + eps;
+  const float maxz = smax(v0.z, smax(v1.z, v2.z)) + eps;
+  const int x0 = max(0, int(floorf(minx * nx)));
+  const int y0 = max(0, int(floorf(miny * ny)));
+  const int z0 = max(0, int(floorf(minz * nz)));
+  const int x1 = min(nx - 1, int(floorf(maxx * nx)));
+  const int y1 = min(ny - 1, int(floorf(maxy * ny)));
+  const int z1 = min(nz - 1, int(floorf(maxz * nz)));
+  if (x1 < x0 || y1 < y0 || z1 < z0) return;
+  const int bx = x1 - x0 + 1, by = y1 - y0 + 1;
+  const long long vol = (long long)bx * by * (z1 - z0 + 1);
+  const F3 half = {0.5f / nx, 0.5f / ny, 0.5f / nz};
+  for (long long v = threadIdx.x; v < vol; v += blockDim.x) {
+    const int dx = int(v % bx);
+    const long long t2 = v / bx;
+    const int dy = int(t2 % by);
+    const int dz = int(t2 / by);
+    const int x = x0 + dx, y = y0 + dy, z = z0 + dz;
+    const F3 c = {(x + 0.5f) / nx, (y + 0.5f) / ny, (z + 0.5f) / nz};
+    if (tri_box(c, half, v0, v1, v2)) surf[(size_t(z) * ny + y) * nx + x] = 1;
+  }
+}
+
+// exterior seeds: non-surface cells on the six faces
+__global__ void __launch_bounds__(256) k_seed(const uint8_t* __restrict__ surf, int nx, int ny,
+                                              int nz, uint8_t* __restrict__ vis) {
+  const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
+  const size_t n = size_t(nx) * ny * nz;
+  if (i >= n) return;
+  const int x = int(i % nx), y = int((i / nx) % ny), z = int(i / (size_t(nx) * ny));
+  const bool face = x == 0 || y == 0 || z == 0 || x == nx - 1 || y == ny - 1 || z == nz - 1;
+  vis[i] = (face && !surf[i]) ? 1 : 0;
+}
+
+// one sweep along axis `axis` per line, both directions
+__global__ void __launch_bounds__(256) k_sweep(const uint8_t* __restrict__ surf,
+                                               uint8_t* __restrict__ vis, int nx, int ny, int nz,
+                                               int axis, int* __restrict__ changed) {
+  const size_t lid = size_t(blockIdx.x) * 256 + threadIdx.x;
+  size_t base, stride;
+  int len;
+  if (axis == 0) {  // lines along x, indexed by (y,z)
+    if (lid >= size_t(ny) * nz) return;
+    base = lid * nx;
+    stride = 1;
+    len = nx;
+  } else if (axis == 1) {  // along y, indexed by (x,z)
+    if (lid >= size_t(nx) * nz) return;
+    const size_t x = lid % nx, z = lid / nx;
+    base = z * size_t(nx) * ny + x;
+    stride = nx;
+    len = ny;
+  } else {  // along z, indexed by (x,y)
+    if (lid >= size_t(nx) * ny) return;
+    base = lid;
+    stride = size_t(nx) * ny;
+    len = nz;
+  }
+  int any = 0;
+  uint8_t prev = 0;
+  for (int k = 0; k < len; ++k) {
+    const size_t i = base + size_t(k) * stride;
+    uint8_t v = vis[i];
+    if (!v && prev && !surf[i]) {
+      v = 1;
+      vis[i] = 1;
+      any = 1;
+    }
+    prev = v;
+  }
+  prev = 0;
+  for (int k = len - 1; k >= 0; --k) {
+    const size_t i = base + size_t(k) * stride;
+    uint8_t v = vis[i];
+    if (!v && prev && !surf[i]) {
+      v = 1;
+      vis[i] = 1;
+      any = 1;
+    }
+    prev = v;
+  }
+  if (any) atomicOr(changed, 1);
+}
+
+__global__ void __launch_bounds__(256) k_finalize(uint8_t* __restrict__ surf_io,
+                                                  const uint8_t* __restrict__ vis, size_t n) {
+  const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  surf_io[i] = (surf_io[i] || !vis[i]) ? 1 : 0;
+}
+
+// downsample_amr (voxelize.cpp:187-226)
+__global__ void __launch_bounds__(256) k_downsample(const uint8_t* __restrict__ fine, int refine,
+                                                    int nx, int ny, int nz, float thr,
+                                                    uint8_t* __restrict__ out) {
+  const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
+  const size_t n = size_t(nx) * ny * nz;
+  if (i >= n) return;
+  const int cx = int(i % nx), cy = int((i / nx) % ny), cz = int(i / (size_t(nx) * ny));
+  const size_t nxf = size_t(nx) * refine, nyf = size_t(ny) * refine;
+  int cnt = 0;
+  for (int dz = 0; dz < refine; ++dz)
+    for (int dy = 0; dy < refine; ++dy) {
+      const size_t b = (size_t(cz) * refine + dz) * nyf * nxf + (size_t(cy) * refine + dy) * nxf +
+                       size_t(cx) * refine;
+      for (int dx = 0; dx < refine; ++dx) cnt += fine[b + dx];
+    }
+  const float cov = float(cnt) / float(refine * refine * refine);
+  out[i] = ((cov > 0.0f) && (cov >= thr)) ? 1 : 0;
+}
+
+// band BFS: dist 0 on solid, -1 elsewhere
+__global__ void __launch_bounds__(256) k_dist_init(const uint8_t* __restrict__ solid, size_t n,
+                                                   int* __restrict__ dist) {
+  const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  dist[i] = solid[i] ? 0 : -1;
+}
+// ring k: unreached cells with a 6-neighbour at k-1 (sim_runner.hpp:57-76)
+__global__ void __launch_bounds__(256) k_dist_ring(int* __restrict__ dist, int nx, int ny, int nz,
+                                                   int k) {
+  const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
+  const size_t n = size_t(nx) * ny * nz;
+  if (i >= n) return;
+  if (dist[i] != -1) return;
+  const int x = int(i % nx), y = int((i / nx) % ny), z = int(i / (size_t(nx) * ny));
+  const size_t sy = nx, sz = size_t(nx) * ny;
+  const int p = k - 1;
+  bool hit = false;
+  if (x > 0) hit |= dist[i - 1] == p;
+  if (x < nx - 1) hit |= dist[i + 1] == p;
+  if (y > 0) hit |= dist[i - sy] == p;
+  if (y < ny - 1) hit |= dist[i + sy] == p;
+  if (z > 0) hit |= dist[i - sz] == p;
+  if (z < nz - 1) hit |= dist[i + sz] == p;
+  if (hit) dist[i] = k;
+}
+__global__ void __launch_bounds__(256) k_band_mask(const uint8_t* __restrict__ solid,
+                                                   const int* __restrict__ dist, size_t n, int d0,
+                                                   uint8_t* __restrict__ band) {
+  const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int d = dist[i];
+  band[i] = (!solid[i] && d >= 0 && d <= d0) ? 1 : 0;
+}
+
+inline dim3 grid1d(size_t n) { return dim3(unsigned((n + 255) / 256)); }
+
+}  // namespace
+
+hipError_t voxelize_device(const TriF* d_tris, int ntri, int nx, int ny, int nz, uint8_t* d_out,
+                           hipStream_t st) {
+  const size_t n = size_t(nx) * ny * nz;
+  hipError_t e = hipMemsetAsync(d_out, 0, n, st);
+  if (e != hipSuccess) return e;
+  if (ntri > 0) {
+    hipLaunchKernelGGL(k_surface, dim3(ntri), dim3(256), 0, st, d_tris, nx, ny, nz, d_out);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  uint8_t* vis = nullptr;
+  int* changed = nullptr;
+  if ((e = hipMalloc((void**)&vis, n)) != hipSuccess) return e;
+  if ((e = hipMalloc((void**)&changed, sizeof(int))) != hipSuccess) {
+    (void)hipFree(vis);
+    return e;
+  }
+  hipLaunchKernelGGL(k_seed, grid1d(n), dim3(256), 0, st, d_out, nx, ny, nz, vis);
+  const size_t lines[3] = {size_t(ny) * nz, size_t(nx) * nz, size_t(nx) * ny};
+  // every round that changes something grows the set; the set has at most n
+  // cells, so the loop ends
+  for (size_t round = 0; round <= n; ++round) {
+    if ((e = hipMemsetAsync(changed, 0, sizeof(int), st)) != hipSuccess) break;
+    for (int ax = 0; ax < 3; ++ax)
+      hipLaunchKernelGGL(k_sweep, grid1d(lines[ax]), dim3(256), 0, st, d_out, vis, nx, ny, nz, ax,
+                         changed);
+    int h = 0;
+    if ((e = hipMemcpyAsync(&h, changed, sizeof(int), hipMemcpyDeviceToHost, st)) != hipSuccess)
+      break;
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) break;
+    if (!h) break;
+  }
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_finalize, grid1d(n), dim3(256), 0, st, d_out, vis, n);
+    e = hipGetLastError();
+  }
+  hipError_t es = hipStreamSynchronize(st);
+  if (e == hipSuccess) e = es;
+  (void)hipFree(vis);
+  (void)hipFree(changed);
+  return e;
+}
+
+hipError_t downsample_device(const uint8_t* d_fine, int refine, int nx, int ny, int nz,
+                             float threshold, uint8_t* d_out, hipStream_t st) {
+  const float thr = fmaxf(0.0f, fminf(1.0f, threshold));
+  const size_t n = size_t(nx) * ny * nz;
+  hipLaunchKernelGGL(k_downsample, grid1d(n), dim3(256), 0, st, d_fine, refine, nx, ny, nz, thr,
+                     d_out);
+  return hipGetLastError();
+}
+
+hipError_t band_device(const uint8_t* d_solid, int nx, int ny, int nz, int d0, uint8_t* d_band,
+                       uint8_t* d_scratch, hipStream_t st) {
+  const size_t n = size_t(nx) * ny * nz;
+  int* dist = reinterpret_cast<int*>(d_scratch);
+  hipLaunchKernelGGL(k_dist_init, grid1d(n), dim3(256), 0, st, d_solid, n, dist);
+  // rings beyond d0 never enter the band, and no ring exceeds the grid's
+  // Manhattan diameter
+  const int rings = d0 < nx + ny + nz ? d0 : nx + ny + nz;
+  for (int k = 1; k <= rings; ++k)
+    hipLaunchKernelGGL(k_dist_ring, grid1d(n), dim3(256), 0, st, dist, nx, ny, nz, k);
+  hipLaunchKernelGGL(k_band_mask, grid1d(n), dim3(256), 0, st, d_solid, dist, n, d0, d_band);
+  return hipGetLastError();
+}
+
+}  // namespace fs

@@ -1,0 +1,236 @@
+"""Python handle on one libfluidsim solver (the reference's Solver concept).
+
+Mirrors LBM3D_Legacy / LBM3D_Home / LBM3D_Hybrid (src/lbm/*.hpp of the
+reference): create (= init), set_solid_mask, set_legacy_mapping, reset,
+reinit_from_macro, step, macro pointers / copies.  Host arrays are numpy;
+device pointers may be passed as ints (e.g. torch ``tensor.data_ptr()``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import capi
+from .capi import BGK, HOME, HYBRID, call, lib
+
+
+def _host_ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class DeviceBuffer:
+    """Device allocation owned by Python (through the library's helpers)."""
+
+    def __init__(self, nbytes: int):
+        p = C.c_void_p()
+        call("fs_device_malloc", C.byref(p), C.c_size_t(nbytes))
+        self.ptr = p.value
+        self.nbytes = nbytes
+
+    @classmethod
+    def from_host(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        call("fs_memcpy_htod", C.c_void_p(b.ptr), _host_ptr(a), C.c_size_t(a.nbytes))
+        return b
+
+    def to_host(self, dtype, count: int) -> np.ndarray:
+        out = np.empty(count, dtype=dtype)
+        call("fs_memcpy_dtoh", _host_ptr(out), C.c_void_p(self.ptr), C.c_size_t(out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().fs_device_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Solver:
+    def __init__(self, method: int, nx: int, ny: int, nz: int, tau: float,
+                 force=(0.0, 0.0, 0.0)):
+        h = C.c_void_p()
+        call("fs_solver_create", int(method), int(nx), int(ny), int(nz), float(tau),
+             float(force[0]), float(force[1]), float(force[2]), C.byref(h))
+        self.h = h
+        self.method = int(method)
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.n = self.nx * self.ny * self.nz
+
+    # -- lifetime
+    def close(self):
+        if self.h:
+            lib().fs_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- configuration
+    def _u8(self, a) -> np.ndarray:
+        a = np.ascontiguousarray(np.asarray(a, dtype=np.uint8).reshape(-1))
+        if a.size != self.n:
+            raise ValueError(f"expected {self.n} bytes, got {a.size}")
+        return a
+
+    def set_solid_mask(self, mask):
+        a = self._u8(mask)
+        call("fs_solver_set_solid_mask", self.h, _host_ptr(a))
+
+    def set_legacy_mapping(self, is_legacy=None):
+        if is_legacy is None:
+            call("fs_solver_set_legacy_mapping", self.h, None, None)
+        else:
+            a = self._u8(is_legacy)
+            call("fs_solver_set_legacy_mapping", self.h, _host_ptr(a), None)
+
+    def set_flags(self, flags):
+        a = self._u8(flags)
+        call("fs_solver_set_flags", self.h, _host_ptr(a))
+
+    def flags(self) -> np.ndarray:
+        out = np.empty(self.n, dtype=np.uint8)
+        call("fs_solver_copy_flags", self.h, _host_ptr(out))
+        return out
+
+    def set_inlet_profile(self, ux=None):
+        if ux is None:
+            call("fs_solver_set_inlet_profile", self.h, None)
+        else:
+            a = np.ascontiguousarray(np.asarray(ux, dtype=np.float32).reshape(-1))
+            if a.size != self.ny * self.nz:
+                raise ValueError("inlet profile needs ny*nz values")
+            call("fs_solver_set_inlet_profile", self.h, _host_ptr(a))
+
+    def set_force(self, fx, fy, fz):
+        call("fs_solver_set_force", self.h, float(fx), float(fy), float(fz))
+
+    def set_omegas(self, omega_legacy, omega_hydro, omega_shear):
+        call("fs_solver_set_omegas", self.h, float(omega_legacy), float(omega_hydro),
+             float(omega_shear))
+
+    # -- state
+    def reset(self):
+        call("fs_solver_reset", self.h)
+
+    def reinit_from_macro(self, rho=None, ux=None, uy=None, uz=None):
+        """Host numpy arrays (copied to the device) or device pointers (int)."""
+        keep = []
+
+        def dev(a):
+            if a is None:
+                return None
+            if isinstance(a, int):
+                return C.c_void_p(a)
+            b = DeviceBuffer.from_host(np.asarray(a, dtype=np.float32).reshape(-1))
+            keep.append(b)
+            return C.c_void_p(b.ptr)
+
+        call("fs_solver_reinit_from_macro", self.h, dev(rho), dev(ux), dev(uy), dev(uz))
+        for b in keep:
+            b.free()
+
+    def step(self, nsteps: int = 1):
+        call("fs_solver_step", self.h, int(nsteps))
+
+    def step_async(self, nsteps: int):
+        call("fs_solver_step_async", self.h, int(nsteps))
+
+    def timed_steps(self, nsteps: int) -> float:
+        """Run nsteps between HIP events on the solver's stream; device ms."""
+        ms = C.c_float()
+        call("fs_solver_timed_steps", self.h, int(nsteps), C.byref(ms))
+        return float(ms.value)
+
+    def synchronize(self):
+        call("fs_solver_synchronize", self.h)
+
+    @property
+    def stream(self) -> int:
+        p = C.c_void_p()
+        call("fs_solver_get_stream", self.h, C.byref(p))
+        return p.value or 0
+
+    def set_stream(self, stream_handle):
+        call("fs_solver_set_stream", self.h, C.c_void_p(stream_handle) if stream_handle else None)
+
+    def macro_ptr(self, which: int) -> int:
+        p = C.c_void_p()
+        call("fs_solver_macro_ptr", self.h, int(which), C.byref(p))
+        return p.value
+
+    def macro(self, which: int) -> np.ndarray:
+        out = np.empty(self.n, dtype=np.float32)
+        call("fs_solver_copy_macro", self.h, int(which), _host_ptr(out))
+        return out
+
+    def macros(self):
+        return [self.macro(k) for k in range(4)]
+
+    def state_size(self) -> int:
+        n = C.c_size_t()
+        call("fs_solver_state_size", self.h, C.byref(n))
+        return n.value
+
+    def state(self) -> np.ndarray:
+        out = np.empty(self.state_size(), dtype=np.float32)
+        call("fs_solver_copy_state", self.h, _host_ptr(out))
+        return out
+
+    def load_state(self, a):
+        a = np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(-1))
+        if a.size != self.state_size():
+            raise ValueError("state size mismatch")
+        call("fs_solver_load_state", self.h, _host_ptr(a))
+
+
+def load_stl(path: str):
+    """load_stl (stl_loader.cpp:102-122): (tri_xyz (T,9) float32, bbmin, bbmax)."""
+    n = C.c_int()
+    bmin = np.zeros(3, np.float32)
+    bmax = np.zeros(3, np.float32)
+    call("fs_load_stl", path.encode(), None, 0, C.byref(n), _host_ptr(bmin), _host_ptr(bmax))
+    tris = np.zeros((max(n.value, 1), 9), np.float32)
+    call("fs_load_stl", path.encode(), _host_ptr(tris), n.value, C.byref(n), _host_ptr(bmin),
+         _host_ptr(bmax))
+    return tris[: n.value], bmin, bmax
+
+
+def voxelize(tris, bbmin, bbmax, nx, ny, nz, uniform_scale=0.5, translate=(0.5, 0.5, 0.5),
+             refine=1, threshold=0.5) -> np.ndarray:
+    """voxelize_mesh_to_mask on the GPU -> uint8 mask of nx*ny*nz (x fastest)."""
+    t = np.ascontiguousarray(np.asarray(tris, np.float32).reshape(-1, 9))
+    bmin = np.ascontiguousarray(np.asarray(bbmin, np.float32))
+    bmax = np.ascontiguousarray(np.asarray(bbmax, np.float32))
+    out = np.empty(nx * ny * nz, np.uint8)
+    call("fs_voxelize", _host_ptr(t), int(t.shape[0]), _host_ptr(bmin), _host_ptr(bmax),
+         int(nx), int(ny), int(nz), float(uniform_scale), float(translate[0]),
+         float(translate[1]), float(translate[2]), int(refine), float(threshold),
+         _host_ptr(out))
+    return out
+
+
+def band(solid, nx, ny, nz, d0) -> np.ndarray:
+    s = np.ascontiguousarray(np.asarray(solid, np.uint8).reshape(-1))
+    out = np.empty(nx * ny * nz, np.uint8)
+    call("fs_band", _host_ptr(s), int(nx), int(ny), int(nz), int(d0), _host_ptr(out))
+    return out
+
+
+__all__ = ["Solver", "DeviceBuffer", "load_stl", "voxelize", "band", "BGK", "HOME", "HYBRID",
+           "capi"]

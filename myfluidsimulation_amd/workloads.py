@@ -1,0 +1,171 @@
+"""Synthetic workloads: 3D Taylor-Green vortex fields, seeded jittered
+icosphere obstacles written as binary STL, and obstacle-channel flag grids.
+
+Random numbers come from splitmix64 (Steele, Lea & Flood, OOPSLA 2014) so
+every consumer (tests, bench, the C++ executables' inputs) gets the same
+geometry from the same seed.
+"""
+from __future__ import annotations
+
+import math
+import struct
+
+import numpy as np
+
+from .capi import FLAG_INLET, FLAG_OUTLET, FLAG_SOLID
+
+_M64 = (1 << 64) - 1
+
+
+class SplitMix64:
+    def __init__(self, seed: int):
+        self.state = seed & _M64
+
+    def next_u64(self) -> int:
+        self.state = (self.state + 0x9E3779B97F4A7C15) & _M64
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & _M64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & _M64
+        return z ^ (z >> 31)
+
+    def uniform(self) -> float:
+        """Uniform double in [0, 1)."""
+        return (self.next_u64() >> 11) * (1.0 / (1 << 53))
+
+
+# ---------------------------------------------------------------- Taylor-Green
+def taylor_green_3d(nx: int, ny: int, nz: int, u0: float = 0.05, rho0: float = 1.0):
+    """3D TGV at cell centres x = i + 0.5 with k = 2*pi/N per axis:
+    u = u0 sin(kx)cos(ky)cos(kz), v = -u0 cos(kx)sin(ky)cos(kz), w = 0,
+    rho = rho0 + (3 rho0 u0^2/16)(cos2kx + cos2ky)(cos2kz + 2).
+    Returns float32 arrays (rho, ux, uy, uz) of nx*ny*nz, x fastest."""
+    z, y, x = np.meshgrid(np.arange(nz) + 0.5, np.arange(ny) + 0.5, np.arange(nx) + 0.5,
+                          indexing="ij")
+    kx, ky, kz = 2 * np.pi / nx, 2 * np.pi / ny, 2 * np.pi / nz
+    X, Y, Z = kx * x, ky * y, kz * z
+    u = u0 * np.sin(X) * np.cos(Y) * np.cos(Z)
+    v = -u0 * np.cos(X) * np.sin(Y) * np.cos(Z)
+    w = np.zeros_like(u)
+    rho = rho0 + (3.0 * rho0 * u0 * u0 / 16.0) * (np.cos(2 * X) + np.cos(2 * Y)) * (
+        np.cos(2 * Z) + 2.0)
+    f32 = lambda a: np.ascontiguousarray(a.reshape(-1), dtype=np.float32)  # noqa: E731
+    return f32(rho), f32(u), f32(v), f32(w)
+
+
+def tgv_energy_analytic(t, u0, tau, n):
+    """Mean kinetic energy 0.5<|u|^2> of the linear-decay 3D TGV on an n^3
+    periodic box: (u0^2/8) exp(-6 nu k^2 t), nu = (tau - 1/2)/3, k = 2pi/n."""
+    nu = (tau - 0.5) / 3.0
+    k = 2 * np.pi / n
+    return (u0 * u0 / 8.0) * np.exp(-6.0 * nu * k * k * np.asarray(t, dtype=np.float64))
+
+
+def kinetic_energy(rho, ux, uy, uz) -> float:
+    """0.5 <|u|^2> over all cells (float64 accumulation)."""
+    u = np.asarray(ux, np.float64)
+    v = np.asarray(uy, np.float64)
+    w = np.asarray(uz, np.float64)
+    return float(0.5 * np.mean(u * u + v * v + w * w))
+
+
+# ---------------------------------------------------------------- icosphere
+def icosphere(subdiv: int = 3):
+    """Unit icosphere: (vertices (V,3) float64, faces (F,3) int), outward
+    counter-clockwise faces; 12 -> 42 -> 162 -> 642 vertices, 20*4^s faces."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t),
+             (0, -1, -t), (0, 1, -t), (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.array(v, float) / np.linalg.norm(v) for v in verts]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
+             (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8),
+             (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdiv):
+        cache = {}
+
+        def mid(a, b):
+            key = (a, b) if a < b else (b, a)
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    return np.array(verts), np.array(faces, dtype=np.int64)
+
+
+def jittered_icosphere(center, radius, seed, jitter=0.10, subdiv=3):
+    """Icosphere with each unique vertex at radius R*(1 + jitter*U[-1,1])
+    (splitmix64 from `seed`, one draw per vertex in vertex order), so the
+    mesh stays closed.  Returns triangle soup (F, 9) float64."""
+    v, f = icosphere(subdiv)
+    rng = SplitMix64(seed)
+    r = np.array([radius * (1.0 + jitter * (2.0 * rng.uniform() - 1.0)) for _ in range(len(v))])
+    p = np.asarray(center, float)[None, :] + v * r[:, None]
+    return p[f].reshape(-1, 9)
+
+
+def write_binary_stl(path: str, tri_xyz) -> None:
+    """Binary STL; the 80-byte header does not start with 'solid' (a file
+    starting with 'solid' is parsed as ASCII first, stl_loader.cpp:108)."""
+    t = np.asarray(tri_xyz, np.float32).reshape(-1, 9)
+    header = b"fluidsim binary stl".ljust(80, b"\0")
+    with open(path, "wb") as fh:
+        fh.write(header)
+        fh.write(struct.pack("<I", t.shape[0]))
+        for tri in t:
+            a, b, c = tri[0:3], tri[3:6], tri[6:9]
+            n = np.cross(b - a, c - a)
+            ln = np.linalg.norm(n)
+            n = n / ln if ln > 0 else n
+            fh.write(struct.pack("<3f", *n.astype(np.float32)))
+            fh.write(tri.astype("<f4").tobytes())
+            fh.write(b"\0\0")
+
+
+def obstacle_stl(path, nx, ny, nz, center, radius, seed):
+    """Write the jittered icosphere in unit-cube coordinates (x/nx, y/ny,
+    z/nz): the voxelizer maps voxel (i,j,k) to ((i+.5)/nx, ...), so the
+    sphere is pre-distorted per axis.  Returns (scale, translate) that make
+    the fit transform the identity: scale = the bbox's largest side (so
+    s = scale/maxDim = 1) and translate = the bbox centre."""
+    soup = jittered_icosphere(center, radius, seed)
+    unit = soup.reshape(-1, 3) / np.array([nx, ny, nz], float)
+    unit = unit.astype(np.float32).reshape(-1, 9)
+    write_binary_stl(path, unit)
+    pts = unit.reshape(-1, 3)
+    bmin, bmax = pts.min(axis=0), pts.max(axis=0)
+    size = bmax - bmin
+    scale = float(np.float32(max(size[0], size[1], size[2])))
+    center32 = np.float32(0.5) * (bmin + bmax)
+    return scale, tuple(float(c) for c in center32)
+
+
+# ---------------------------------------------------------------- channel
+def channel_flags(nx, ny, nz, solid=None):
+    """Flag grid of an obstacle channel: solid y/z wall planes OR-ed with the
+    obstacle, equilibrium inlet cells at x=0, zero-gradient outlet cells at
+    x=nx-1 (fluid cells only; an outlet needs a fluid cell at x-1)."""
+    fl = np.zeros((nz, ny, nx), np.uint8)
+    if solid is not None:
+        fl |= (np.asarray(solid, np.uint8).reshape(nz, ny, nx) != 0).astype(np.uint8) * FLAG_SOLID
+    fl[:, 0, :] |= FLAG_SOLID
+    fl[:, ny - 1, :] |= FLAG_SOLID
+    fl[0, :, :] |= FLAG_SOLID
+    fl[nz - 1, :, :] |= FLAG_SOLID
+    fluid = (fl & FLAG_SOLID) == 0
+    fl[:, :, 0] |= np.where(fluid[:, :, 0], FLAG_INLET, 0).astype(np.uint8)
+    ok_out = fluid[:, :, nx - 1] & fluid[:, :, nx - 2]
+    fl[:, :, nx - 1] |= np.where(ok_out, FLAG_OUTLET, 0).astype(np.uint8)
+    return fl.reshape(-1)
+
+
+def uniform_flow(n, u_in):
+    rho = np.ones(n, np.float32)
+    ux = np.full(n, u_in, np.float32)
+    z = np.zeros(n, np.float32)
+    return rho, ux, z, z.copy()

@@ -1,0 +1,121 @@
+"""GPU geometry and field helpers against the CPU oracle: voxelizer (bit for
+bit), band BFS (bit for bit), |u|, density profiles, the stand-alone feq
+re-initialisation."""
+import os
+
+import numpy as np
+import pytest
+
+from myfluidsimulation_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def _stl(tmp_path, nx, ny, nz, center, radius, seed):
+    path = os.path.join(tmp_path, f"ico_{seed}.stl")
+    scale, trans = wl.obstacle_stl(path, nx, ny, nz, center, radius, seed)
+    return path, scale, trans
+
+
+@pytest.mark.parametrize("refine", [1, 2, 3])
+def test_voxelize_icosphere_exact(fs, oracle, tmp_path, refine):
+    nx, ny, nz = 96, 32, 32
+    path, scale, trans = _stl(str(tmp_path), nx, ny, nz, (24, 16, 16), 6.0, 11 + refine)
+    tris, bmin, bmax = fs.load_stl(path)
+    otris, obmin, obmax = oracle.load_stl(path)
+    np.testing.assert_array_equal(tris, otris)
+    np.testing.assert_array_equal(bmin, obmin)
+    np.testing.assert_array_equal(bmax, obmax)
+    g = fs.voxelize(tris, bmin, bmax, nx, ny, nz, scale, trans, refine, 0.5)
+    o = oracle.voxelize(otris, obmin, obmax, nx, ny, nz, scale, trans, refine, 0.5)
+    assert g.sum() > 0
+    assert np.array_equal(g, o), f"{int((g != o).sum())} voxels differ"
+
+
+def test_voxelize_reference_defaults_exact(fs, oracle, tmp_path):
+    """Default VoxelParams path (scale 0.5 about the centre) on a non-cubic
+    grid, plus thresholds 0.25 / 1.0 for the AMR downsample."""
+    path = os.path.join(str(tmp_path), "s.stl")
+    wl.write_binary_stl(path, wl.jittered_icosphere((0, 0, 0), 1.0, 5))
+    tris, bmin, bmax = fs.load_stl(path)
+    for thr in (0.25, 1.0):
+        g = fs.voxelize(tris, bmin, bmax, 40, 30, 20, 0.5, (0.5, 0.5, 0.5), 2, thr)
+        o = oracle.voxelize(tris, bmin, bmax, 40, 30, 20, 0.5, (0.5, 0.5, 0.5), 2, thr)
+        assert np.array_equal(g, o)
+    g = fs.voxelize(tris, bmin, bmax, 40, 30, 20)
+    o = oracle.voxelize(tris, bmin, bmax, 40, 30, 20)
+    assert np.array_equal(g, o)
+
+
+def test_voxelize_channel_config(fs, oracle, tmp_path):
+    """The obstacle of the 384x128x128 channel: jittered icosphere, radius
+    20 cells, centre (96,64,64), 1280 faces."""
+    nx, ny, nz = 384, 128, 128
+    path, scale, trans = _stl(str(tmp_path), nx, ny, nz, (96, 64, 64), 20.0, 2024)
+    tris, bmin, bmax = fs.load_stl(path)
+    assert tris.shape == (1280, 9)
+    g = fs.voxelize(tris, bmin, bmax, nx, ny, nz, scale, trans)
+    o = oracle.voxelize(tris, bmin, bmax, nx, ny, nz, scale, trans)
+    assert np.array_equal(g, o)
+    vol = 4.0 / 3.0 * np.pi * 20.0 ** 3
+    assert 0.7 * vol < g.sum() < 1.4 * vol
+
+
+@pytest.mark.parametrize("d0", [0, 1, 2, 3, 5])
+def test_band_exact(fs, oracle, d0):
+    nx, ny, nz = 40, 24, 20
+    rng = np.random.default_rng(d0)
+    solid = (rng.random(nx * ny * nz) < 0.02).astype(np.uint8)
+    solid.reshape(nz, ny, nx)[:, 0, :] = 1
+    g = fs.band(solid, nx, ny, nz, d0)
+    o = oracle.band(solid, nx, ny, nz, d0)
+    assert np.array_equal(g, o)
+
+
+def test_band_no_solid(fs):
+    assert fs.band(np.zeros(8 * 8 * 8, np.uint8), 8, 8, 8, 3).sum() == 0
+
+
+def test_fields_speed_and_profiles(fs, oracle):
+    import ctypes as C
+
+    from myfluidsimulation_amd import DeviceBuffer
+    from myfluidsimulation_amd.capi import call
+
+    nx, ny, nz = 30, 20, 10
+    n = nx * ny * nz
+    rng = np.random.default_rng(1)
+    u, v, w = (rng.standard_normal(n).astype(np.float32) for _ in range(3))
+    bu, bv, bw = (DeviceBuffer.from_host(a) for a in (u, v, w))
+    out = DeviceBuffer(4 * n)
+    call("fs_speed", C.c_void_p(bu.ptr), C.c_void_p(bv.ptr), C.c_void_p(bw.ptr),
+         C.c_void_p(out.ptr), n)
+    np.testing.assert_array_equal(out.to_host(np.float32, n), oracle.speed(u, v, w))
+    call("fs_rho_slab", C.c_void_p(out.ptr), nx, ny, nz, 1, 2e-3, 0.4, 0.3)
+    np.testing.assert_array_equal(out.to_host(np.float32, n),
+                                  oracle.rho_slab(nx, ny, nz, 1, 2e-3, 0.4, 0.3))
+    call("fs_rho_gauss", C.c_void_p(out.ptr), nx, ny, nz, 0.6, 0.5, 0.5, 0.08, 1e-3)
+    # expf on the GPU vs std::exp on the host: tolerance, not bits
+    np.testing.assert_allclose(out.to_host(np.float32, n),
+                               oracle.rho_gauss(nx, ny, nz, 0.6, 0.5, 0.5, 0.08, 1e-3),
+                               rtol=0, atol=2e-7)
+
+
+def test_reinit_equilibrium_reference_entry(fs, oracle):
+    """reinit_equilibrium_from_macro (lbm_reinit.cu:52-61) writes f[q*N+i] with
+    the cs2 form: equal to the Legacy solver's reinit state."""
+    import ctypes as C
+
+    from myfluidsimulation_amd import DeviceBuffer
+    from myfluidsimulation_amd.capi import lib
+
+    nx, ny, nz = 16, 12, 8
+    n = nx * ny * nz
+    rho, ux, uy, uz = wl.taylor_green_3d(nx, ny, nz, 0.05)
+    bufs = [DeviceBuffer.from_host(a) for a in (rho, ux, uy, uz)]
+    f = DeviceBuffer(19 * 4 * n)
+    lib().reinit_equilibrium_from_macro(*[C.c_void_p(b.ptr) for b in bufs], C.c_void_p(f.ptr), n)
+    lib().fs_device_synchronize()
+    o = oracle.Oracle(0, nx, ny, nz, 0.6)
+    o.reinit(rho, ux, uy, uz)
+    np.testing.assert_array_equal(f.to_host(np.float32, 19 * n), o.state())

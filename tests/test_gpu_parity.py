@@ -1,0 +1,201 @@
+"""GPU parity: libfluidsim.so (HIP, through the C ABI) against the FP32 CPU
+oracle on identical float32 inputs.
+
+The per-cell arithmetic of the kernels is the reference rule's, in the same
+float order and without contraction, so fields are expected to be equal bit
+for bit; the acceptance bar stated for this path is a relative max-norm of
+1e-5 on rho, u (and HOME's moments) after 1000 steps (TOL below).
+"""
+import numpy as np
+import pytest
+
+from myfluidsimulation_amd import workloads as wl
+
+TOL = 1e-5  # relative max-norm, rho/u/moments vs the FP32 oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_maxnorm(got, ref):
+    got = np.asarray(got, np.float64)
+    ref = np.asarray(ref, np.float64)
+    scale = max(np.max(np.abs(ref)), 1e-30)
+    return float(np.max(np.abs(got - ref)) / scale)
+
+
+def run_pair(fs, oracle, method, dims, tau, steps, flags=None, fields=None, inlet=None,
+             force=(0.0, 0.0, 0.0), substeps_split=None):
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    s = fs.Solver(method, nx, ny, nz, tau, force)
+    o = oracle.Oracle(method, nx, ny, nz, tau, force)
+    if flags is not None:
+        s.set_flags(flags)
+        o.set_flags(flags)
+    if inlet is not None:
+        s.set_inlet_profile(inlet)
+        o.set_inlet(inlet)
+    if fields is None:
+        fields = (np.ones(n, np.float32), None, None, None)
+    s.reinit_from_macro(*fields)
+    o.reinit(*fields)
+    chunks = substeps_split or [steps]
+    for k in chunks:
+        s.step(k)
+    o.step(steps)
+    out = dict(g_macro=s.macros(), o_macro=o.macros(), g_state=s.state(), o_state=o.state())
+    s.close()
+    o.close()
+    return out
+
+
+def assert_exact(res, what=""):
+    for k, (g, r) in enumerate(zip(res["g_macro"], res["o_macro"])):
+        assert np.array_equal(g, r), f"{what} macro {k}: rel {rel_maxnorm(g, r):.3e}"
+    assert np.array_equal(res["g_state"], res["o_state"]), (
+        f"{what} state: rel {rel_maxnorm(res['g_state'], res['o_state']):.3e}")
+
+
+@pytest.mark.parametrize("method", [0, 1, 2], ids=["bgk", "home", "hybrid"])
+def test_tgv64_1000_steps(fs, oracle, method):
+    """Taylor-Green 64^3, tau 0.6, u0 0.05, 1000 steps (the CPU-runnable case)."""
+    n = 64
+    fields = wl.taylor_green_3d(n, n, n, 0.05)
+    res = run_pair(fs, oracle, method, (n, n, n), 0.6, 1000, fields=fields)
+    errs = [rel_maxnorm(g, r) for g, r in zip(res["g_macro"][:3], res["o_macro"][:3])]
+    print(f"method {method}: rel max-norm rho/ux/uy {errs}")
+    assert max(errs) <= TOL
+    if method == 1:
+        assert rel_maxnorm(res["g_state"], res["o_state"]) <= TOL
+    assert_exact(res, f"tgv method {method}")
+
+
+@pytest.mark.parametrize("method", [0, 1, 2], ids=["bgk", "home", "hybrid"])
+@pytest.mark.parametrize("steps", [1, 2, 7, 40])
+def test_obstacle_box_exact(fs, oracle, method, steps):
+    """Periodic box with a solid block and a solid column touching the
+    periodic seam: bounce-back, solid macros, both AA parities."""
+    nx, ny, nz = 32, 24, 16
+    solid = np.zeros((nz, ny, nx), np.uint8)
+    solid[5:11, 8:15, 9:17] = 1
+    solid[:, 0, 0] = 1
+    solid[3, :, 31] = 1
+    rng = np.random.default_rng(7)
+    n = nx * ny * nz
+    rho = (1.0 + 0.01 * rng.standard_normal(n)).astype(np.float32)
+    ux = (0.02 * rng.standard_normal(n)).astype(np.float32)
+    uy = (0.02 * rng.standard_normal(n)).astype(np.float32)
+    uz = (0.02 * rng.standard_normal(n)).astype(np.float32)
+    res = run_pair(fs, oracle, method, (nx, ny, nz), 0.7, steps, flags=solid.reshape(-1),
+                   fields=(rho, ux, uy, uz), force=(1e-5, -2e-6, 0.0))
+    assert_exact(res, f"box method {method} steps {steps}")
+
+
+def test_step_split_invariance(fs, oracle):
+    """step(3)+step(4)+step(5) equals step(12): macros are written on the
+    last step of each call and the AA parity carries across calls."""
+    nx, ny, nz = 24, 16, 8
+    solid = np.zeros((nz, ny, nx), np.uint8)
+    solid[2:5, 4:9, 6:12] = 1
+    fields = wl.taylor_green_3d(nx, ny, nz, 0.04)
+    for method in (0, 2):
+        res = run_pair(fs, oracle, method, (nx, ny, nz), 0.8, 12, flags=solid.reshape(-1),
+                       fields=fields, substeps_split=[3, 4, 5])
+        assert_exact(res, f"split method {method}")
+
+
+@pytest.mark.parametrize("method", [0, 1, 2], ids=["bgk", "home", "hybrid"])
+def test_channel_inlet_outlet_band(fs, oracle, method):
+    """Obstacle channel: y/z walls, equilibrium inlet, zero-gradient outlet,
+    a box obstacle, the Hybrid band (d0 = 2) from the CPU band rule."""
+    nx, ny, nz = 48, 20, 18
+    obst = np.zeros((nz, ny, nx), np.uint8)
+    obst[6:12, 7:13, 12:18] = 1
+    flags = wl.channel_flags(nx, ny, nz, obst.reshape(-1))
+    solid = (flags & 1).astype(np.uint8)
+    band = oracle.band(solid, nx, ny, nz, 2)
+    flags = flags | (band * 2).astype(np.uint8)
+    inlet = np.full(ny * nz, 0.05, np.float32)
+    fields = wl.uniform_flow(nx * ny * nz, 0.05)
+    for steps in (1, 2, 301):
+        res = run_pair(fs, oracle, method, (nx, ny, nz), 0.56, steps, flags=flags,
+                       fields=fields, inlet=inlet)
+        assert_exact(res, f"channel method {method} steps {steps}")
+
+
+def test_hybrid_equals_legacy_when_rates_equal(fs):
+    """omega_s = omega_h (lbm3d_hybrid.cu:264-267) makes every Hybrid cell
+    a BGK cell: with u = 0 inits the two solvers agree bit for bit."""
+    nx, ny, nz = 32, 16, 16
+    solid = np.zeros((nz, ny, nx), np.uint8)
+    solid[4:10, 5:11, 8:16] = 1
+    out = []
+    for method in (0, 2):
+        with fs.Solver(method, nx, ny, nz, 0.6, (1e-5, 0, 0)) as s:
+            s.set_solid_mask(solid.reshape(-1))
+            if method == 2:
+                s.set_legacy_mapping((solid.reshape(-1) == 0).astype(np.uint8))
+            s.reinit_from_macro(np.ones(nx * ny * nz, np.float32))
+            s.step(50)
+            out.append(s.macros())
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
+
+
+def test_hybrid_two_rate_form(fs, oracle):
+    """A shear rate different from the hydro rate exercises the two-rate
+    bulk form; the band cells keep BGK."""
+    nx, ny, nz = 24, 16, 12
+    solid = np.zeros((nz, ny, nx), np.uint8)
+    solid[3:7, 5:10, 6:12] = 1
+    band = oracle.band(solid.reshape(-1), nx, ny, nz, 1)
+    flags = solid.reshape(-1) | (band * 2)
+    fields = wl.taylor_green_3d(nx, ny, nz, 0.03)
+    s = fs.Solver(2, nx, ny, nz, 0.6)
+    o = oracle.Oracle(2, nx, ny, nz, 0.6)
+    for x in (s, o):
+        x.set_flags(flags)
+    s.set_omegas(1 / 0.6, 1 / 0.6, 1.4)
+    o.set_omegas(float(np.float32(1 / 0.6)), float(np.float32(1 / 0.6)), float(np.float32(1.4)))
+    s.reinit_from_macro(*fields)
+    o.reinit(*fields)
+    s.step(25)
+    o.step(25)
+    for g, r in zip(s.macros(), o.macros()):
+        assert np.array_equal(g, r)
+
+
+def test_reset_and_null_reinit(fs, oracle):
+    nx, ny, nz = 16, 12, 10
+    for method in (0, 1, 2):
+        with fs.Solver(method, nx, ny, nz, 0.7) as s:
+            s.reinit_from_macro(None)  # Legacy: reset(); HOME/Hybrid: rho = 1
+            s.step(3)
+            o = oracle.Oracle(method, nx, ny, nz, 0.7)
+            o.reinit(None)
+            o.step(3)
+            for g, r in zip(s.macros(), o.macros()):
+                assert np.array_equal(g, r)
+            s.reset()
+            np.testing.assert_array_equal(s.macro(0), np.ones(nx * ny * nz, np.float32))
+
+
+def test_load_state_roundtrip(fs, oracle):
+    nx, ny, nz = 20, 12, 8
+    rng = np.random.default_rng(3)
+    for method in (0, 1):
+        with fs.Solver(method, nx, ny, nz, 0.65) as s:
+            fields = wl.taylor_green_3d(nx, ny, nz, 0.05)
+            s.reinit_from_macro(*fields)
+            s.step(5)
+            st = s.state()
+            s.load_state(st)
+            np.testing.assert_array_equal(s.state(), st)
+            # an arbitrary state steps like the oracle from the same state
+            pert = (st * (1 + 1e-3 * rng.standard_normal(st.size))).astype(np.float32)
+            s.load_state(pert)
+            o = oracle.Oracle(method, nx, ny, nz, 0.65)
+            o.set_state(pert)
+            s.step(3)
+            o.step(3)
+            np.testing.assert_array_equal(s.state(), o.state())
