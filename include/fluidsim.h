@@ -52,6 +52,17 @@ extern "C" {
 #define FS_FLAG_INLET 0x04u  /* extension: equilibrium inlet cell */
 #define FS_FLAG_OUTLET 0x08u /* extension: zero-gradient outlet cell (x = nx-1) */
 
+/* ---- bounce-back partner rule (fs_solver_set_bounce_rule) ----
+ * HALFWAY: a population hitting a solid cell returns in the opposite
+ *   direction (e = -e_q): mass-conserving halfway bounce-back, all methods.
+ * REFERENCE_TABLE: the partner from the reference's opp table
+ *   (lbm3d_legacy.cu:29-32), which for q >= 7 pairs mirror images
+ *   (e.g. (+1,+1,0) with (-1,+1,0)).  HOME only: its gather stays
+ *   deterministic; in the populations solvers' push update that table
+ *   writes some slots twice and others never. */
+#define FS_BOUNCE_HALFWAY 0
+#define FS_BOUNCE_REFERENCE_TABLE 1
+
 /* ---- macro selectors for fs_solver_macro_ptr / fs_solver_copy_macro ---- */
 #define FS_RHO 0
 #define FS_UX 1
@@ -101,6 +112,10 @@ int fs_solver_set_force(fs_solver* s, float fx, float fy, float fz);
  * 1/tau, which is the default here).  FS_ERR_STATE for other methods. */
 int fs_solver_set_omegas(fs_solver* s, float omega_legacy, float omega_hydro,
                          float omega_shear);
+
+/* Extension: bounce-back partner rule (FS_BOUNCE_*; default HALFWAY).
+ * FS_ERR_STATE for REFERENCE_TABLE on BGK / Hybrid. */
+int fs_solver_set_bounce_rule(fs_solver* s, int rule);
 
 /* reset() (lbm3d_legacy.cu:153-159, lbm3d_home.cu:313-319,
  * lbm3d_hybrid.cu:237-243). */

@@ -30,6 +30,9 @@ FLAG_OUTLET = 0x08
 
 RHO, UX, UY, UZ = 0, 1, 2, 3
 
+BOUNCE_HALFWAY = 0
+BOUNCE_REFERENCE_TABLE = 1
+
 _p = C.c_void_p
 _f = C.c_float
 _i = C.c_int
@@ -53,6 +56,7 @@ SIGNATURES = {
     "fs_solver_set_inlet_profile": (_i, [_p, _p]),
     "fs_solver_set_force": (_i, [_p, _f, _f, _f]),
     "fs_solver_set_omegas": (_i, [_p, _f, _f, _f]),
+    "fs_solver_set_bounce_rule": (_i, [_p, _i]),
     "fs_solver_reset": (_i, [_p]),
     "fs_solver_reinit_from_macro": (_i, [_p, _p, _p, _p, _p]),
     "fs_solver_step": (_i, [_p, _i]),

@@ -235,7 +235,7 @@ int fs_solver_create(int method, int nx, int ny, int nz, float tau, float fx, fl
     s->g = fs::make_geo(nx, ny, nz);
     s->tau = tau;
     const float omega = 1.0f / tau;
-    s->ph = {omega, omega, omega, fx, fy, fz};
+    s->ph = {omega, omega, omega, fx, fy, fz, FS_BOUNCE_HALFWAY};
     s->hflags.assign(s->g.n, 0);
     auto bail = [&](int code) {
       free_solver(s);
@@ -355,6 +355,18 @@ int fs_solver_set_omegas(fs_solver* s, float oL, float oH, float oS) {
   s->ph.omegaL = oL;
   s->ph.omegaH = oH;
   s->ph.omegaS = oS;
+  return FS_OK;
+}
+
+int fs_solver_set_bounce_rule(fs_solver* s, int rule) {
+  if (int rc = check_solver(s)) return rc;
+  if (rule != FS_BOUNCE_HALFWAY && rule != FS_BOUNCE_REFERENCE_TABLE)
+    return fail(FS_ERR_INVALID, "unknown bounce rule %d", rule);
+  if (rule == FS_BOUNCE_REFERENCE_TABLE && s->method != fs::kMethodHOME)
+    return fail(FS_ERR_STATE,
+                "the reference pairing table is not a permutation of the populations; only "
+                "HOME's gather can use it");
+  s->ph.bounce = rule;
   return FS_OK;
 }
 

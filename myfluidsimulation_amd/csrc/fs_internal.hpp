@@ -20,7 +20,15 @@ namespace fs {
 constexpr int CX[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
 constexpr int CY[19] = {0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1};
 constexpr int CZ[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1};
-constexpr int OPP[19] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
+// Opposite direction, e_OPP[q] = -e_q: the halfway bounce-back partner and
+// the AA storage partner.
+constexpr int OPP[19] = {0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 16, 15};
+// The reference's own table (lbm3d_legacy.cu:29-32, lbm3d_home.cu:27-30,
+// lbm3d_hybrid.cu:28-31).  For q >= 7 it pairs mirror images (one velocity
+// component flipped, e.g. 7=(+1,+1,0) with 8=(-1,+1,0)), not opposites.
+// Only HOME's bounce-back can use it (FS_BOUNCE_REFERENCE_TABLE): there the
+// update is a gather and stays deterministic.
+constexpr int OPP_REF[19] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
 
 // Flag byte: the low four bits are the public FS_FLAG_* bits; the upper bits
 // are derived on the device after every flag upload.
@@ -88,6 +96,7 @@ inline Geo make_geo(int nx, int ny, int nz) {
 struct Phys {
   float omegaL, omegaH, omegaS;  // BGK / hybrid-bulk rates (all 1/tau by default)
   float fx, fy, fz;              // velocity shift u_eq = u + F
+  int bounce;                    // 0: opposite direction, 1: reference table (HOME)
 };
 
 // ---- launchers (defined in the .hip files; all enqueue on `st`) ----

@@ -4,9 +4,10 @@
 // swap per step (lbm3d_home.cu:288-354).  A cell gathers, for each of the 19
 // directions, the regularised post-collision population of its upstream
 // neighbour, rebuilt from that neighbour's moments (reconstruct_f_post,
-// lbm3d_home.cu:37-81), or its own opposite population when the neighbour is
-// solid (halfway bounce-back, lbm3d_home.cu:219-225), and accumulates the new
-// moments (lbm3d_home.cu:238-285).  Per-direction arithmetic is specialised
+// lbm3d_home.cu:37-81), or one of its own populations when the neighbour is
+// solid (halfway bounce-back, lbm3d_home.cu:219-225: the opposite direction
+// by default, the reference's own pairing table on request), and accumulates
+// the new moments (lbm3d_home.cu:238-285).  Per-direction arithmetic is specialised
 // at compile time (d3q19_dev.hpp) and keeps the reference's float order.
 #include "d3q19_dev.hpp"
 
@@ -20,7 +21,7 @@ __device__ __forceinline__ void load_mom(const float* __restrict__ m, size_t P, 
   for (int k = 0; k < 10; ++k) v.v[k] = m[size_t(k) * P + s];
 }
 
-template <bool CHECK>
+template <bool CHECK, int BB>
 __device__ __forceinline__ void home_gather(const float* __restrict__ m, size_t P,
                                             const uint8_t* __restrict__ flags, uint32_t cc,
                                             const Nbr& o, const Phys& ph, float oneMinusOmega,
@@ -38,7 +39,7 @@ __device__ __forceinline__ void home_gather(const float* __restrict__ m, size_t 
     } else {
       const uint32_t s = cc + nb_off<-a, -b, -c>(o);
       if (CHECK && (flags[s] & F_SOLID)) {
-        fq = home_rec<OPP[q]>(mc, oneMinusOmega);
+        fq = home_rec<BB == 0 ? OPP[q] : OPP_REF[q]>(mc, oneMinusOmega);
       } else {
         Mom ms;
         load_mom(m, P, s, ms);
@@ -76,6 +77,7 @@ __device__ __forceinline__ void home_gather(const float* __restrict__ m, size_t 
   out[9] = Mzz - (rcs2 + (r * uz) * uz);
 }
 
+template <int BB>
 __global__ void __launch_bounds__(256) k_home_step(Geo g, Phys ph, HomeLaunch a) {
   Cell cl;
   if (!cell_of_thread(g, cl)) return;
@@ -101,9 +103,9 @@ __global__ void __launch_bounds__(256) k_home_step(Geo g, Phys ph, HomeLaunch a)
     const Nbr o = nbr_offsets(xc, cl.y, cl.z, g);
     const float omm = 1.0f - ph.omegaL;
     if (__ballot((flc & F_NEAR) != 0) == 0)
-      home_gather<false>(a.m, P, a.flags, cc, o, ph, omm, out);
+      home_gather<false, BB>(a.m, P, a.flags, cc, o, ph, omm, out);
     else
-      home_gather<true>(a.m, P, a.flags, cc, o, ph, omm, out);
+      home_gather<true, BB>(a.m, P, a.flags, cc, o, ph, omm, out);
   }
 #pragma unroll
   for (int k = 0; k < 10; ++k) a.mn[size_t(k) * P + c] = out[k];
@@ -130,7 +132,10 @@ inline dim3 lattice_grid(const Geo& g) { return dim3((g.nxy + 255) / 256, g.nz, 
 }  // namespace
 
 hipError_t launch_home_step(const Geo& g, const Phys& ph, const HomeLaunch& a, hipStream_t st) {
-  hipLaunchKernelGGL(k_home_step, lattice_grid(g), dim3(256), 0, st, g, ph, a);
+  if (ph.bounce == 0)
+    hipLaunchKernelGGL(k_home_step<0>, lattice_grid(g), dim3(256), 0, st, g, ph, a);
+  else
+    hipLaunchKernelGGL(k_home_step<1>, lattice_grid(g), dim3(256), 0, st, g, ph, a);
   return hipGetLastError();
 }
 

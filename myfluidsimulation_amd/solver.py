@@ -124,6 +124,9 @@ class Solver:
         call("fs_solver_set_omegas", self.h, float(omega_legacy), float(omega_hydro),
              float(omega_shear))
 
+    def set_bounce_rule(self, rule: int):
+        call("fs_solver_set_bounce_rule", self.h, int(rule))
+
     # -- state
     def reset(self):
         call("fs_solver_reset", self.h)

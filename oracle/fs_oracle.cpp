@@ -50,7 +50,12 @@ namespace {
 constexpr int CX[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
 constexpr int CY[19] = {0, 0, 0, 1, -1, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 1, -1, 1, -1};
 constexpr int CZ[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1};
-constexpr int OPP[19] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
+// The reference's opp table (lbm3d_legacy.cu:29-32).  For q >= 7 it pairs
+// mirror images, not opposites: e_{OPP_REF[7]} = (-1,+1,0) while -e_7 =
+// (-1,-1,0).
+constexpr int OPP_REF[19] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
+// Opposite directions (e_OPP[q] = -e_q): the halfway bounce-back partner.
+constexpr int OPP[19] = {0, 2, 1, 4, 3, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 16, 15};
 
 template <class T>
 struct Weights {
@@ -122,6 +127,7 @@ struct SolverBase {
   virtual int set_flags(const uint8_t* flags) = 0;
   virtual int set_inlet(const float* ux) = 0;
   virtual int set_omegas(double a, double b, double c) = 0;
+  virtual int set_bounce(int rule) = 0;
   virtual int reset() = 0;
   virtual int reinit_f32(const float*, const float*, const float*, const float*) = 0;
   virtual int step(int n) = 0;
@@ -196,6 +202,7 @@ struct PopSolver : SolverBase {
     for (size_t i = 0; i < inlet.size(); ++i) inlet[i] = T(u[i]);
     return 0;
   }
+  int set_bounce(int rule) override { return rule == 0 ? 0 : -1; }
   int set_omegas(double a, double b, double c) override {
     omegaL = T(a);
     omegaH = T(b);
@@ -370,6 +377,7 @@ template <class T>
 struct HomeSolver : SolverBase {
   Grid g;
   T tau, fx, fy, fz, omega;
+  int bounce = 0;  // 0: opposite direction, 1: the reference's table
   Weights<T> W;
   std::vector<T> m, mnext;
   std::vector<uint8_t> flags;
@@ -399,6 +407,10 @@ struct HomeSolver : SolverBase {
     return 0;
   }
   int set_omegas(double, double, double) override { return -1; }
+  int set_bounce(int rule) override {
+    bounce = rule;
+    return 0;
+  }
 
   // kern_reset_mom (lbm3d_home.cu:83-107)
   int reset() override {
@@ -439,7 +451,7 @@ struct HomeSolver : SolverBase {
       const size_t ids = g.nb(x, y, z, -CX[q], -CY[q], -CZ[q]);
       T fq;
       if (flags[ids] & ORC_SOLID) {
-        fq = home_rec(W.w, OPP[q], rhoC, uxC0, uyC0, uzC0, SxxC, SxyC, SxzC, SyyC, SyzC, SzzC,
+        fq = home_rec(W.w, bounce ? OPP_REF[q] : OPP[q], rhoC, uxC0, uyC0, uzC0, SxxC, SxyC, SxzC, SyyC, SyzC, SzzC,
                       oneMinusOmega);
       } else {
         fq = home_rec(W.w, q, A[0 * N + ids], A[1 * N + ids], A[2 * N + ids], A[3 * N + ids],
@@ -811,7 +823,32 @@ int orc_set_inlet(orc_solver* s, const float* ux) { return s->impl->set_inlet(ux
 int orc_set_omegas(orc_solver* s, double a, double b, double c) {
   return s->impl->set_omegas(a, b, c);
 }
+int orc_set_bounce(orc_solver* s, int rule) { return s->impl->set_bounce(rule); }
 int orc_reset(orc_solver* s) { return s->impl->reset(); }
+
+// Writers per fnext slot of one push step (kern_collide_stream's store
+// pattern, lbm3d_legacy.cu:92-108) with the given partner table: a valid
+// streaming + bounce-back update writes every fluid slot exactly once.
+int orc_push_write_counts(const uint8_t* solid, int nx, int ny, int nz, int table,
+                          int32_t* counts) {
+  Grid g{nx, ny, nz, (size_t)nx * ny * nz};
+  const int* opp = table ? OPP_REF : OPP;
+  std::memset(counts, 0, sizeof(int32_t) * 19 * g.n);
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        const size_t id = g.id(x, y, z);
+        if (solid[id]) continue;
+        for (int q = 0; q < 19; ++q) {
+          const size_t id2 = g.nb(x, y, z, CX[q], CY[q], CZ[q]);
+          if (solid[id2])
+            counts[opp[q] * g.n + id] += 1;
+          else
+            counts[q * g.n + id2] += 1;
+        }
+      }
+  return 0;
+}
 int orc_reinit_f32(orc_solver* s, const float* r, const float* u, const float* v, const float* w) {
   return s->impl->reinit_f32(r, u, v, w);
 }

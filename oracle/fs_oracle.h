@@ -51,7 +51,14 @@ int orc_set_inlet(orc_solver* s, const float* ux);
 int orc_set_omegas(orc_solver* s, double omega_legacy, double omega_hydro,
                    double omega_shear);
 
+/* bounce-back partner: 0 opposite direction, 1 the reference's opp table
+ * (HOME only; the populations solvers reject 1) */
+int orc_set_bounce(orc_solver* s, int rule);
 int orc_reset(orc_solver* s);
+/* writers per slot of one push step with partner table 0 (opposite) or 1
+ * (the reference's); counts is 19*N int32, solid cells write nothing */
+int orc_push_write_counts(const uint8_t* solid, int nx, int ny, int nz,
+                          int table, int32_t* counts);
 /* nullable pointers follow each method's reference semantics */
 int orc_reinit_f32(orc_solver* s, const float* rho, const float* ux,
                    const float* uy, const float* uz);

@@ -44,6 +44,8 @@ def lib():
             "orc_set_inlet": (_i, [_p, _p]),
             "orc_set_omegas": (_i, [_p, _d, _d, _d]),
             "orc_reset": (_i, [_p]),
+            "orc_set_bounce": (_i, [_p, _i]),
+            "orc_push_write_counts": (_i, [_p, _i, _i, _i, _i, _p]),
             "orc_reinit_f32": (_i, [_p, _p, _p, _p, _p]),
             "orc_step": (_i, [_p, _i]),
             "orc_get_macro_f32": (_i, [_p, _i, _p]),
@@ -125,6 +127,9 @@ class Oracle:
     def reset(self):
         _chk(lib().orc_reset(self.h), "reset")
 
+    def set_bounce(self, rule):
+        _chk(lib().orc_set_bounce(self.h, int(rule)), "set_bounce")
+
     def reinit(self, rho=None, ux=None, uy=None, uz=None):
         r, u, v, w = _f32(rho), _f32(ux), _f32(uy), _f32(uz)
         _chk(lib().orc_reinit_f32(self.h, _ptr(r), _ptr(u), _ptr(v), _ptr(w)), "reinit")
@@ -188,6 +193,14 @@ def distance(solid, nx, ny, nz, max_dist):
     s = np.ascontiguousarray(np.asarray(solid, np.uint8).reshape(-1))
     out = np.empty(nx * ny * nz, np.int32)
     _chk(lib().orc_distance(_ptr(s), nx, ny, nz, int(max_dist), _ptr(out)), "distance")
+    return out
+
+
+def push_write_counts(solid, nx, ny, nz, table):
+    """Writers per populations slot in one push step (19*N int32)."""
+    s = np.ascontiguousarray(np.asarray(solid, np.uint8).reshape(-1))
+    out = np.empty(19 * nx * ny * nz, np.int32)
+    _chk(lib().orc_push_write_counts(_ptr(s), nx, ny, nz, int(table), _ptr(out)), "counts")
     return out
 
 
