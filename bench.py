@@ -1,24 +1,30 @@
 """Benchmark: D3Q19 lattice updates per second on one MI355X.
 
-Workload (the configuration the headline metric is quoted on): obstacle
-channel 384x128x128, FP32, tau 0.56, inlet u=0.05 at x=0 (equilibrium),
-zero-gradient outlet at x=383, solid y/z wall planes, and a seeded jittered
-icosphere (3 subdivisions, 1280 faces, radius 20 cells, centre (96,64,64))
-written as binary STL and voxelized on the GPU.  BGK (the headline `value`),
-HOME and Hybrid(B0) band 2 each run W untimed then K timed steps; one "step"
-is one lattice time step over the whole domain.
+Default workload (the configuration the headline metric is quoted on):
+obstacle channel 384x128x128, FP32, tau 0.56, inlet u=0.05 at x=0
+(equilibrium), zero-gradient outlet at x=383, solid y/z wall planes, and a
+seeded jittered icosphere (3 subdivisions, 1280 faces, radius 20 cells,
+centre (96,64,64)) written as binary STL and voxelized on the GPU.  BGK (the
+headline `value`), HOME and Hybrid(B0) band 2 each run W untimed then K
+timed steps; one "step" is one lattice time step over the whole domain.
+
+Other workloads (--workload), the large single-domain roofline runs:
+  tgv512      512^3 periodic Taylor-Green vortex, BGK and HOME
+  channel768  768x256x256 channel, icosphere radius 40, Hybrid bands 1, 2, 4
+              (and BGK, HOME)
 
 Timing: HIP events recorded on the solver's stream around the K steps
 (fs_solver_timed_steps); every step is one kernel launch, so the event time
-divided by K is the dominant kernel's average launch duration.
+divided by K is the step kernel's average launch duration.
 
 roofline.achieved uses algorithmic bytes per lattice update (B/LUP): BGK and
-Hybrid 152 (19 floats read + 19 written, in place), HOME 80 (10 + 10), counted
-over fluid cells only (solid cells move no data); peak = 8.0 TB/s HBM3E.
+Hybrid 152 (19 floats read + 19 written, in place), HOME 80 (10 + 10),
+counted over fluid cells only (solid cells move no data); peak = 8.0 TB/s
+HBM3E.  MLUPS counts every cell, solid included.
 
 cpu_baseline: the FP32 CPU oracle (scalar C++ restatement of the reference
 rule, OpenMP over z-planes, all host threads) timed on a bounded sample of
-the same channel (a few BGK steps).
+the same workload (a few BGK steps).
 """
 from __future__ import annotations
 
@@ -35,48 +41,76 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBPS = 8000.0
-B_LUP = {"bgk": 152.0, "home": 80.0, "hybrid": 152.0}
-NX, NY, NZ, TAU, U_IN = 384, 128, 128, 0.56, 0.05
+B_LUP = {0: 152.0, 1: 80.0, 2: 152.0}
+NAMES = {0: "bgk", 1: "home", 2: "hybrid"}
 SEED = 2024
-BAND = 2
 
 
-def build_channel():
-    """Flag grid of the channel: voxelized icosphere + walls + inlet/outlet."""
+class Case:
+    """One domain: size, tau, flag grid, initial fields, inlet profile."""
+
+    def __init__(self, nx, ny, nz, tau, flags, fields, inlet=None, solid=None):
+        self.nx, self.ny, self.nz, self.tau = nx, ny, nz, tau
+        self.flags, self.fields, self.inlet = flags, fields, inlet
+        self.solid = solid
+
+    @property
+    def n(self):
+        return self.nx * self.ny * self.nz
+
+    def fluid(self):
+        return self.n if self.flags is None else int(((self.flags & 1) == 0).sum())
+
+
+def channel_case(nx, ny, nz, center, radius, tau=0.56, u_in=0.05):
     from myfluidsimulation_amd import solver as S
     from myfluidsimulation_amd import workloads as wl
 
     with tempfile.TemporaryDirectory() as td:
         path = os.path.join(td, "obstacle.stl")
-        scale, trans = wl.obstacle_stl(path, NX, NY, NZ, (96, 64, 64), 20.0, SEED)
+        scale, trans = wl.obstacle_stl(path, nx, ny, nz, center, radius, SEED)
         tris, bmin, bmax = S.load_stl(path)
-    solid = S.voxelize(tris, bmin, bmax, NX, NY, NZ, scale, trans)
-    flags = wl.channel_flags(NX, NY, NZ, solid)
-    band = S.band((flags & 1).astype(np.uint8), NX, NY, NZ, BAND)
-    return flags, band
+    solid = S.voxelize(tris, bmin, bmax, nx, ny, nz, scale, trans)
+    flags = wl.channel_flags(nx, ny, nz, solid)
+    return Case(nx, ny, nz, tau, flags, wl.uniform_flow(nx * ny * nz, u_in),
+                np.full(ny * nz, u_in, np.float32), (flags & 1).astype(np.uint8))
 
 
-def run_method(method, flags, band, steps, warmup):
-    from myfluidsimulation_amd import Solver
+def tgv_case(n, tau=0.6, u0=0.05):
     from myfluidsimulation_amd import workloads as wl
 
-    n = NX * NY * NZ
-    fl = flags | (band * 2).astype(np.uint8) if method == 2 else flags
-    s = Solver(method, NX, NY, NZ, TAU)
-    s.set_flags(fl)
-    s.set_inlet_profile(np.full(NY * NZ, U_IN, np.float32))
-    s.reinit_from_macro(*wl.uniform_flow(n, U_IN))
+    return Case(n, n, n, tau, None, wl.taylor_green_3d(n, n, n, u0))
+
+
+def run_method(case, method, steps, warmup, band=0):
+    from myfluidsimulation_amd import Solver
+    from myfluidsimulation_amd import solver as S
+
+    s = Solver(method, case.nx, case.ny, case.nz, case.tau)
+    if case.flags is not None:
+        fl = case.flags
+        if method == 2 and band > 0:
+            b = S.band(case.solid, case.nx, case.ny, case.nz, band)
+            fl = fl | (b * 2).astype(np.uint8)
+        s.set_flags(fl)
+    if case.inlet is not None:
+        s.set_inlet_profile(case.inlet)
+    s.reinit_from_macro(*case.fields)
     if warmup:
         s.step(warmup)
     ms = s.timed_steps(steps)
-    rho = s.macro(0)
-    ok = bool(np.isfinite(rho).all())
+    ok = bool(np.isfinite(s.macro(0)).all())
     s.close()
-    return ms, ok
+    per = ms / steps
+    gbps = case.fluid() * B_LUP[method] / (per * 1e-3) / 1e9
+    return {"MLUPS": round(case.n * steps / (ms * 1e-3) / 1e6, 1),
+            "ms_per_step": round(per, 5), "B_per_LUP": B_LUP[method],
+            "achieved_GBps": round(gbps, 1), "roofline_frac": round(gbps / PEAK_HBM_GBPS, 4),
+            "finite": ok}
 
 
 def copy_ceiling():
-    """Measured HBM copy ceiling (GB/s): 2 GiB streaming copy."""
+    """Measured HBM copy ceiling (GB/s): 2 GiB streaming copy, 10 launches."""
     import ctypes as C
 
     from myfluidsimulation_amd import DeviceBuffer
@@ -92,18 +126,17 @@ def copy_ceiling():
     return 2.0 * nbytes * 10 / (ms.value * 1e-3) / 1e9
 
 
-def cpu_baseline(flags, budget_s=12.0):
-    """FP32 oracle (BGK) on the same channel, OpenMP on all host threads."""
+def cpu_baseline(case, budget_s=12.0):
+    """FP32 oracle (BGK) on the same workload, OpenMP on all host threads."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle  # the checker, used here only as the timed CPU baseline
 
-    from myfluidsimulation_amd import workloads as wl
-
-    n = NX * NY * NZ
-    o = pyoracle.Oracle(0, NX, NY, NZ, TAU)
-    o.set_flags(flags)
-    o.set_inlet(np.full(NY * NZ, U_IN, np.float32))
-    o.reinit(*wl.uniform_flow(n, U_IN))
+    o = pyoracle.Oracle(0, case.nx, case.ny, case.nz, case.tau)
+    if case.flags is not None:
+        o.set_flags(case.flags)
+    if case.inlet is not None:
+        o.set_inlet(case.inlet)
+    o.reinit(*case.fields)
     t0 = time.perf_counter()
     o.step(1)
     one = time.perf_counter() - t0
@@ -112,17 +145,18 @@ def cpu_baseline(flags, budget_s=12.0):
     o.step(k)
     dt = time.perf_counter() - t0
     o.close()
-    return {"value": round(n * k / dt / 1e6, 2), "unit": "MLUPS",
+    return {"value": round(case.n * k / dt / 1e6, 2), "unit": "MLUPS",
             "cores": pyoracle.num_threads(), "kind": "port",
-            "sample": f"{k} BGK steps of the {NX}x{NY}x{NZ} channel (FP32 oracle, "
-                      f"{dt:.1f} s)"}
+            "sample": f"{k} BGK steps of the {case.nx}x{case.ny}x{case.nz} workload "
+                      f"(FP32 oracle, OpenMP, {dt:.1f} s)"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5000)
-    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--workload", default="channel", choices=["channel", "tgv512", "channel768"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -137,47 +171,55 @@ def main():
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
 
-    flags, band = build_channel()
-    n = NX * NY * NZ
-    fluid = int(((flags & 1) == 0).sum())
+    if args.workload == "channel":
+        case = channel_case(384, 128, 128, (96, 64, 64), 20.0)
+        runs = [("bgk", 0, 0), ("home", 1, 0), ("hybrid", 2, 2)]
+        steps, warmup = args.steps or 5000, 100 if args.warmup is None else args.warmup
+        desc = ("obstacle channel 384x128x128, tau 0.56, inlet u 0.05, zero-gradient outlet, "
+                f"y/z walls, icosphere R20 seed {SEED}; value = BGK, also HOME and Hybrid band 2")
+    elif args.workload == "tgv512":
+        case = tgv_case(512)
+        runs = [("bgk", 0, 0), ("home", 1, 0)]
+        steps, warmup = args.steps or 200, 50 if args.warmup is None else args.warmup
+        desc = "Taylor-Green vortex 512^3 periodic, tau 0.6, u0 0.05; value = BGK, also HOME"
+    else:
+        case = channel_case(768, 256, 256, (192, 128, 128), 40.0)
+        runs = [("bgk", 0, 0), ("home", 1, 0), ("hybrid_band1", 2, 1),
+                ("hybrid_band2", 2, 2), ("hybrid_band4", 2, 4)]
+        steps, warmup = args.steps or 200, 50 if args.warmup is None else args.warmup
+        desc = ("obstacle channel 768x256x256, tau 0.56, icosphere R40; value = BGK, also HOME "
+                "and Hybrid bands 1, 2, 4")
+
     methods = {}
-    for name, m in (("bgk", 0), ("home", 1), ("hybrid", 2)):
-        ms, ok = run_method(m, flags, band, args.steps, args.warmup)
-        per = ms / args.steps
-        mlups = n * args.steps / (ms * 1e-3) / 1e6
-        gbps = fluid * B_LUP[name] / (per * 1e-3) / 1e9
-        methods[name] = {"MLUPS": round(mlups, 1), "ms_per_step": round(per, 5),
-                         "B_per_LUP": B_LUP[name], "achieved_GBps": round(gbps, 1),
-                         "roofline_frac": round(gbps / PEAK_HBM_GBPS, 4), "finite": ok}
-        print(f"# {name}: {mlups:.1f} MLUPS, {per * 1e3:.1f} us/step, {gbps:.0f} GB/s",
-              file=sys.stderr, flush=True)
+    for name, m, band in runs:
+        methods[name] = run_method(case, m, steps, warmup, band)
+        r = methods[name]
+        print(f"# {name}: {r['MLUPS']:.1f} MLUPS, {r['ms_per_step'] * 1e3:.1f} us/step, "
+              f"{r['achieved_GBps']:.0f} GB/s", file=sys.stderr, flush=True)
     ceiling = copy_ceiling()
-    bgk = methods["bgk"]
+    head = methods[runs[0][0]]
     line = {
         "metric": "MLUPS & % HBM roofline (BGK/HOME/hybrid); batched & adjoint MLUPS vs plain",
-        "value": bgk["MLUPS"],
+        "value": head["MLUPS"],
         "unit": "MLUPS",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": bgk["ms_per_step"],
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded jittered-icosphere obstacle channel)",
-        "config": {"workload": f"obstacle channel {NX}x{NY}x{NZ}, tau {TAU}, inlet u {U_IN}, "
-                               f"zero-gradient outlet, y/z walls, icosphere R20 seed {SEED}; "
-                               f"value = BGK, also HOME and Hybrid band {BAND}",
-                   "cells": n, "fluid_cells": fluid},
+        "data": "synthetic (seeded)",
+        "config": {"workload": desc, "cells": case.n, "fluid_cells": case.fluid()},
         "methods": methods,
-        "roofline": {"bound": "hbm", "achieved": bgk["achieved_GBps"], "peak": PEAK_HBM_GBPS,
-                     "unit": "GB/s", "frac": bgk["roofline_frac"], "traffic": None,
+        "roofline": {"bound": "hbm", "achieved": head["achieved_GBps"], "peak": PEAK_HBM_GBPS,
+                     "unit": "GB/s", "frac": head["roofline_frac"], "traffic": None,
                      "kernel": "k_pop_step (BGK, AA pattern)",
                      "measured_copy_ceiling_GBps": round(ceiling, 1)},
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(flags)
+        line["cpu_baseline"] = cpu_baseline(case)
     print(json.dumps(line))
 
 

@@ -10,7 +10,7 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libfluidsim.so")
+LIB_PATH = os.environ.get("FLUIDSIM_LIB") or os.path.join(HERE, "lib", "libfluidsim.so")
 ABI_VERSION = 1
 
 FS_OK = 0

@@ -150,15 +150,52 @@ __device__ __forceinline__ int nb_off(const Nbr& o) {
   return s;
 }
 
-// Cell coordinates for the (plane-linear, z) launch grid.
+#ifndef FS_NT
+#define FS_NT 0
+#endif
+#ifndef FS_XCD
+#define FS_XCD 0
+#endif
+
+// Population loads and stores of the streaming kernels (non-temporal when
+// FS_NT: every value is touched once per step, so caching it buys nothing).
+__device__ __forceinline__ float ld_pop(const float* p) {
+#if FS_NT
+  return __builtin_nontemporal_load(p);
+#else
+  return *p;
+#endif
+}
+__device__ __forceinline__ void st_pop(float* p, float v) {
+#if FS_NT
+  __builtin_nontemporal_store(v, p);
+#else
+  *p = v;
+#endif
+}
+
+// Cell coordinates for the (plane-linear, z) launch grid.  With FS_XCD the
+// workgroup order is remapped so the workgroups the dispatcher sends to one
+// XCD (every 8th) cover one contiguous slab of cells, keeping the partial
+// cache lines shared by neighbouring workgroups inside one L2.
 struct Cell {
   uint32_t c;
   int x, y, z;
 };
 __device__ __forceinline__ bool cell_of_thread(const Geo& g, Cell& cl) {
-  const uint32_t cp = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t bx = blockIdx.x, bz = blockIdx.y;
+#if FS_XCD
+  const uint32_t total = gridDim.x * gridDim.y;
+  if ((total & 7u) == 0) {
+    const uint32_t L = blockIdx.y * gridDim.x + blockIdx.x;
+    const uint32_t logical = (L & 7u) * (total >> 3) + (L >> 3);
+    bz = logical / gridDim.x;
+    bx = logical - bz * gridDim.x;
+  }
+#endif
+  const uint32_t cp = bx * blockDim.x + threadIdx.x;
   if (cp >= g.nxy) return false;
-  cl.z = blockIdx.y;
+  cl.z = int(bz);
   const uint32_t y = g.divx.div(cp);
   cl.y = int(y);
   cl.x = int(cp - y * uint32_t(g.nx));

@@ -90,9 +90,9 @@ __device__ __forceinline__ void emit_odd(float* __restrict__ F, size_t P,
     constexpr int q = decltype(qc)::value;
     const uint32_t d = c + nb_off<CX[q], CY[q], CZ[q]>(o);
     if (CHECK && q != 0 && (flags[d] & F_SOLID)) {
-      F[size_t(OPP[q]) * P + c] = f[q];
+      st_pop(&F[size_t(OPP[q]) * P + c], f[q]);
     } else {
-      F[size_t(q) * P + d] = f[q];
+      st_pop(&F[size_t(q) * P + d], f[q]);
     }
   });
 }
@@ -131,22 +131,22 @@ __global__ void __launch_bounds__(256) k_pop_step(Geo g, Phys ph, PopLaunch a) {
     if (!ODD) {
       for_q<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        f[q] = F[size_t(q) * P + c];
+        f[q] = ld_pop(&F[size_t(q) * P + c]);
       });
     } else if (!anyNear) {
       for_q<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         const uint32_t s = c + nb_off<-CX[q], -CY[q], -CZ[q]>(o);
-        f[q] = F[size_t(OPP[q]) * P + s];
+        f[q] = ld_pop(&F[size_t(OPP[q]) * P + s]);
       });
     } else {
       for_q<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         const uint32_t s = c + nb_off<-CX[q], -CY[q], -CZ[q]>(o);
         if (q != 0 && (a.flags[s] & F_SOLID))
-          f[q] = F[size_t(q) * P + c];
+          f[q] = ld_pop(&F[size_t(q) * P + c]);
         else
-          f[q] = F[size_t(OPP[q]) * P + s];
+          f[q] = ld_pop(&F[size_t(OPP[q]) * P + s]);
       });
     }
     collide<HYB>(f, r, ux, uy, uz, ph, (fl & F_BAND) != 0);
@@ -155,12 +155,12 @@ __global__ void __launch_bounds__(256) k_pop_step(Geo g, Phys ph, PopLaunch a) {
   if (!ODD) {
     for_q<0, 19>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
-      F[size_t(OPP[q]) * P + c] = f[q];
+      st_pop(&F[size_t(OPP[q]) * P + c], f[q]);
     });
     if (fl & F_FEEDER) {
       for_q<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        F[size_t(OPP[q]) * P + c + 1] = f[q];
+        st_pop(&F[size_t(OPP[q]) * P + c + 1], f[q]);
       });
     }
   } else {
