@@ -115,6 +115,54 @@ __device__ __forceinline__ float home_rec(const Mom& m, float oneMinusOmega) {
   return feq + oneMinusOmega * fneq;
 }
 
+// One source cell's shared terms of reconstruct_f_post: everything that does
+// not depend on the direction (u.u, w*rho per weight class, the two possible
+// Q_aa*S_aa products per axis) is computed once; emit<Q>() then finishes
+// direction Q with exactly the operations home_rec<Q> performs, so the result
+// has the same bits.
+struct HomeSrc {
+  float ux, uy, uz, uu15, wr0, wr1, wr7;
+  float pA[3], pB[3];  // (1 - 1/3)*S_aa and (0 - 1/3)*S_aa for a = x, y, z
+  float sxy, sxz, syz, omm;
+
+  __device__ __forceinline__ HomeSrc(const Mom& m, float oneMinusOmega) {
+    constexpr float cs2 = 1.0f / 3.0f;
+    constexpr float A = 1.0f - cs2, B = 0.0f - cs2;
+    const float rho = m.v[0];
+    ux = m.v[1];
+    uy = m.v[2];
+    uz = m.v[3];
+    uu15 = 1.5f * (ux * ux + uy * uy + uz * uz);
+    wr0 = Dir<0>::w * rho;
+    wr1 = Dir<1>::w * rho;
+    wr7 = Dir<7>::w * rho;
+    pA[0] = A * m.v[4];
+    pB[0] = B * m.v[4];
+    pA[1] = A * m.v[7];
+    pB[1] = B * m.v[7];
+    pA[2] = A * m.v[9];
+    pB[2] = B * m.v[9];
+    sxy = m.v[5];
+    sxz = m.v[6];
+    syz = m.v[8];
+    omm = oneMinusOmega;
+  }
+
+  template <int Q>
+  __device__ __forceinline__ float emit() const {
+    constexpr int a = Dir<Q>::cx, b = Dir<Q>::cy, c = Dir<Q>::cz;
+    constexpr float w45 = 4.5f * Dir<Q>::w;
+    const float wr = Q == 0 ? wr0 : (Q <= 6 ? wr1 : wr7);
+    const float eu = edot<Q>(ux, uy, uz);
+    const float feq = wr * (((1.0f + 3.0f * eu) + (4.5f * eu) * eu) - uu15);
+    float contraction = ((a ? pA[0] : pB[0]) + (b ? pA[1] : pB[1])) + (c ? pA[2] : pB[2]);
+    if constexpr (a != 0 && b != 0) contraction += 2.0f * sg<a * b>(sxy);
+    if constexpr (a != 0 && c != 0) contraction += 2.0f * sg<a * c>(sxz);
+    if constexpr (b != 0 && c != 0) contraction += 2.0f * sg<b * c>(syz);
+    return feq + omm * (w45 * contraction);
+  }
+};
+
 // Compile-time loop over the 19 directions.
 template <int Q, int END, class Fn>
 __device__ __forceinline__ void for_q(Fn&& fn) {

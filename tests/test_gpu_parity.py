@@ -199,3 +199,18 @@ def test_load_state_roundtrip(fs, oracle):
             s.step(3)
             o.step(3)
             np.testing.assert_array_equal(s.state(), o.state())
+
+
+@pytest.mark.parametrize("dims", [(65, 12, 10), (130, 9, 17), (64, 8, 16)])
+def test_home_tile_shapes(fs, oracle, dims):
+    """HOME's tiled kernel on partial tiles and z-chunks, and the plain
+    gather it falls back to when nx-1 starts a 64-wide tile (nx = 65)."""
+    nx, ny, nz = dims
+    obst = np.zeros((nz, ny, nx), np.uint8)
+    obst[2:5, 3:6, 10:14] = 1
+    flags = wl.channel_flags(nx, ny, nz, obst.reshape(-1))
+    inlet = np.full(ny * nz, 0.04, np.float32)
+    fields = wl.uniform_flow(nx * ny * nz, 0.04)
+    res = run_pair(fs, oracle, 1, (nx, ny, nz), 0.62, 9, flags=flags, fields=fields, inlet=inlet,
+                   force=(1e-6, 0.0, 0.0))
+    assert_exact(res, f"home tile {dims}")
