@@ -140,6 +140,17 @@ int fs_solver_synchronize(fs_solver* s);
  * stream, wait, and return the elapsed device time in milliseconds. */
 int fs_solver_timed_steps(fs_solver* s, int nsteps, float* elapsed_ms);
 
+/* Extension: kernel variant selection.  HOME has a plain per-cell gather
+ * (variant 0) and tiled kernels (1: 64x4 columns x 16 planes, 2: 64x8 x 16,
+ * 3: 64x8 x 32); every variant gives the same bits.  -1 restores the
+ * per-grid default.  BGK / Hybrid have one kernel (variant 0).
+ * fs_solver_autotune times each applicable variant for trial_steps steps
+ * from the current state (restored after each trial) and keeps the
+ * fastest; *chosen (may be NULL) receives it. */
+int fs_solver_set_variant(fs_solver* s, int variant);
+int fs_solver_get_variant(const fs_solver* s, int* variant);
+int fs_solver_autotune(fs_solver* s, int trial_steps, int* chosen);
+
 /* d_rho() d_u() d_v() d_w() (lbm3d_legacy.cu:229-236, lbm3d_home.cu:355-362):
  * borrowed device pointer to N floats.  BGK and Hybrid report the moments of
  * the populations at the start of the last step (the reference's one-step

@@ -65,6 +65,9 @@ SIGNATURES = {
     "fs_solver_get_stream": (_i, [_p, C.POINTER(_p)]),
     "fs_solver_synchronize": (_i, [_p]),
     "fs_solver_timed_steps": (_i, [_p, _i, C.POINTER(_f)]),
+    "fs_solver_set_variant": (_i, [_p, _i]),
+    "fs_solver_get_variant": (_i, [_p, C.POINTER(_i)]),
+    "fs_solver_autotune": (_i, [_p, _i, C.POINTER(_i)]),
     "fs_solver_macro_ptr": (_i, [_p, _i, C.POINTER(_p)]),
     "fs_solver_copy_macro": (_i, [_p, _i, _p]),
     "fs_solver_state_size": (_i, [_p, C.POINTER(_sz)]),

@@ -82,6 +82,7 @@ struct fs_solver {
   // moments (HOME)
   float* mbuf[2] = {nullptr, nullptr};
   int cur = 0;
+  int home_variant = -1;  // HOME kernel variant, -1 = default for the grid
   // flags
   uint8_t* flags = nullptr;  // derived flag byte used by the kernels
   uint8_t* stage = nullptr;  // public flag bytes uploaded from the host
@@ -173,7 +174,7 @@ int enqueue_steps(fs_solver* s, int nsteps) {
   if (s->method == fs::kMethodHOME) {
     for (int i = 0; i < nsteps; ++i) {
       fs::HomeLaunch a{s->mbuf[s->cur], s->mbuf[1 - s->cur], s->flags, s->inlet};
-      FS_HIP(fs::launch_home_step(s->g, s->ph, a, s->stream));
+      FS_HIP(fs::launch_home_step(s->g, s->ph, a, s->home_variant, s->stream));
       s->cur = 1 - s->cur;
     }
   } else {
@@ -446,6 +447,63 @@ int fs_solver_timed_steps(fs_solver* s, int nsteps, float* elapsed_ms) {
   (void)hipEventDestroy(e1);
   if (rc != FS_OK) return rc;
   if (e != hipSuccess) return fail(FS_ERR_DEVICE, "timed steps: %s", hipGetErrorString(e));
+  return FS_OK;
+}
+
+int fs_solver_set_variant(fs_solver* s, int variant) {
+  if (int rc = check_solver(s)) return rc;
+  if (s->method != fs::kMethodHOME) return variant <= 0 ? FS_OK : fail(FS_ERR_INVALID, "no such variant");
+  if (variant >= 0 && !fs::home_variant_ok(s->g, variant))
+    return fail(FS_ERR_INVALID, "HOME variant %d does not apply to this grid", variant);
+  s->home_variant = variant;
+  return FS_OK;
+}
+
+int fs_solver_get_variant(const fs_solver* s, int* variant) {
+  if (int rc = check_solver(s)) return rc;
+  if (!variant) return fail(FS_ERR_INVALID, "null output");
+  *variant = s->method == fs::kMethodHOME
+                 ? (s->home_variant >= 0 ? s->home_variant : fs::home_default_variant(s->g))
+                 : 0;
+  return FS_OK;
+}
+
+int fs_solver_autotune(fs_solver* s, int trial_steps, int* chosen) {
+  if (int rc = check_solver(s)) return rc;
+  if (trial_steps < 1) return fail(FS_ERR_INVALID, "trial_steps must be >= 1");
+  if (s->method != fs::kMethodHOME) {
+    if (chosen) *chosen = 0;
+    return FS_OK;
+  }
+  // time every applicable variant from the current state, restoring it after
+  // each trial, and keep the fastest
+  const size_t bytes = sizeof(float) * 10 * s->g.P;
+  float* saved = nullptr;
+  FS_HIP(hipMalloc((void**)&saved, bytes));
+  const int cur0 = s->cur;
+  int best = -1;
+  float best_ms = 0.0f;
+  int rc = FS_OK;
+  hipError_t e = hipMemcpyAsync(saved, s->mbuf[cur0], bytes, hipMemcpyDeviceToDevice, s->stream);
+  for (int v = 0; v < fs::kHomeVariants && e == hipSuccess && rc == FS_OK; ++v) {
+    if (!fs::home_variant_ok(s->g, v)) continue;
+    s->home_variant = v;
+    float ms = 0.0f;
+    rc = enqueue_steps(s, 1);  // warm-up (code load, caches)
+    if (rc == FS_OK) rc = fs_solver_timed_steps(s, trial_steps, &ms);
+    if (rc == FS_OK && (best < 0 || ms < best_ms)) {
+      best = v;
+      best_ms = ms;
+    }
+    e = hipMemcpyAsync(s->mbuf[cur0], saved, bytes, hipMemcpyDeviceToDevice, s->stream);
+    s->cur = cur0;
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  (void)hipFree(saved);
+  if (rc != FS_OK) return rc;
+  if (e != hipSuccess) return fail(FS_ERR_DEVICE, "autotune: %s", hipGetErrorString(e));
+  s->home_variant = best;
+  if (chosen) *chosen = best;
   return FS_OK;
 }
 

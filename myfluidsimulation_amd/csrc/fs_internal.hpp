@@ -124,7 +124,14 @@ struct HomeLaunch {
   const uint8_t* flags;
   const float* inlet;
 };
-hipError_t launch_home_step(const Geo& g, const Phys& ph, const HomeLaunch& a, hipStream_t st);
+// HOME kernel variants: the plain per-cell gather and three tile shapes
+// (64 x TY columns, ZC-plane z-chunks)
+constexpr int kHomePlain = 0, kHomeT4Z16 = 1, kHomeT8Z16 = 2, kHomeT8Z32 = 3, kHomeVariants = 4;
+bool home_variant_ok(const Geo& g, int variant);
+int home_default_variant(const Geo& g);
+// variant < 0: the default for this grid
+hipError_t launch_home_step(const Geo& g, const Phys& ph, const HomeLaunch& a, int variant,
+                            hipStream_t st);
 hipError_t launch_home_fill(const Geo& g, float* m, float* mn, const float* rho, const float* ux,
                             const float* uy, const float* uz, hipStream_t st);
 

@@ -342,26 +342,45 @@ __global__ void __launch_bounds__(256) k_home_fill(Geo g, float* m, float* mn, c
 
 inline dim3 lattice_grid(const Geo& g) { return dim3((g.nxy + 255) / 256, g.nz, 1); }
 
+template <int TY, int ZC>
+hipError_t launch_tiled(const Geo& g, const Phys& ph, const HomeLaunch& a, hipStream_t st) {
+  constexpr int TX = 64;
+  const unsigned blocks =
+      unsigned(((g.nx + TX - 1) / TX) * ((g.ny + TY - 1) / TY) * ((g.nz + ZC - 1) / ZC));
+  if (ph.bounce == 0)
+    hipLaunchKernelGGL((k_home_tiled<TX, TY, ZC, 0>), dim3(blocks), dim3(TX * TY), 0, st, g, ph, a);
+  else
+    hipLaunchKernelGGL((k_home_tiled<TX, TY, ZC, 1>), dim3(blocks), dim3(TX * TY), 0, st, g, ph, a);
+  return hipGetLastError();
+}
+
 }  // namespace
 
-#ifndef FS_HOME_SIMPLE
-#define FS_HOME_SIMPLE 0
-#endif
+bool home_variant_ok(const Geo& g, int variant) {
+  if (variant == kHomePlain) return true;
+  if (variant < kHomePlain || variant >= kHomeVariants) return false;
+  // the tiled kernels take an outlet's moments from the lane at x-1, which
+  // must sit in the same 64-wide tile: not so when nx-1 starts a tile
+  return (g.nx - 1) % 64 != 0;
+}
 
-hipError_t launch_home_step(const Geo& g, const Phys& ph, const HomeLaunch& a, hipStream_t st) {
-  constexpr int TX = 64, TY = 8, ZC = 16;
-  // the tiled kernel takes an outlet's moments from the lane at x-1, which
-  // must sit in the same tile: not so when nx-1 starts a tile
-  if (!FS_HOME_SIMPLE && (g.nx - 1) % TX != 0) {
-    const unsigned blocks = unsigned(((g.nx + TX - 1) / TX) * ((g.ny + TY - 1) / TY) *
-                                     ((g.nz + ZC - 1) / ZC));
-    if (ph.bounce == 0)
-      hipLaunchKernelGGL((k_home_tiled<TX, TY, ZC, 0>), dim3(blocks), dim3(TX * TY), 0, st, g, ph,
-                         a);
-    else
-      hipLaunchKernelGGL((k_home_tiled<TX, TY, ZC, 1>), dim3(blocks), dim3(TX * TY), 0, st, g, ph,
-                         a);
-    return hipGetLastError();
+int home_default_variant(const Geo& g) {
+  if (!home_variant_ok(g, kHomeT8Z32)) return kHomePlain;
+  // deep chunks cut the re-computed halo planes, but only pay when they
+  // still leave enough workgroups (>= 4 per CU) to fill the chip
+  const long tiles8 = long((g.nx + 63) / 64) * ((g.ny + 7) / 8);
+  if (g.nz >= 128 && tiles8 * ((g.nz + 31) / 32) >= 1024) return kHomeT8Z32;
+  return kHomeT4Z16;
+}
+
+hipError_t launch_home_step(const Geo& g, const Phys& ph, const HomeLaunch& a, int variant,
+                            hipStream_t st) {
+  if (variant < 0 || !home_variant_ok(g, variant)) variant = home_default_variant(g);
+  switch (variant) {
+    case kHomeT4Z16: return launch_tiled<4, 16>(g, ph, a, st);
+    case kHomeT8Z16: return launch_tiled<8, 16>(g, ph, a, st);
+    case kHomeT8Z32: return launch_tiled<8, 32>(g, ph, a, st);
+    default: break;
   }
   if (ph.bounce == 0)
     hipLaunchKernelGGL(k_home_step<0>, lattice_grid(g), dim3(256), 0, st, g, ph, a);

@@ -160,6 +160,21 @@ class Solver:
         call("fs_solver_timed_steps", self.h, int(nsteps), C.byref(ms))
         return float(ms.value)
 
+    def set_variant(self, variant: int):
+        call("fs_solver_set_variant", self.h, int(variant))
+
+    @property
+    def variant(self) -> int:
+        v = C.c_int()
+        call("fs_solver_get_variant", self.h, C.byref(v))
+        return v.value
+
+    def autotune(self, trial_steps: int = 10) -> int:
+        """Time every kernel variant from the current state; keep the fastest."""
+        v = C.c_int()
+        call("fs_solver_autotune", self.h, int(trial_steps), C.byref(v))
+        return v.value
+
     def synchronize(self):
         call("fs_solver_synchronize", self.h)
 

@@ -214,3 +214,33 @@ def test_home_tile_shapes(fs, oracle, dims):
     res = run_pair(fs, oracle, 1, (nx, ny, nz), 0.62, 9, flags=flags, fields=fields, inlet=inlet,
                    force=(1e-6, 0.0, 0.0))
     assert_exact(res, f"home tile {dims}")
+
+
+def test_home_variants_and_autotune(fs, oracle):
+    """Every HOME kernel variant gives the oracle's bits; autotune leaves the
+    state untouched and picks one of them."""
+    nx, ny, nz = 130, 20, 40
+    obst = np.zeros((nz, ny, nx), np.uint8)
+    obst[10:20, 6:12, 30:40] = 1
+    flags = wl.channel_flags(nx, ny, nz, obst.reshape(-1))
+    inlet = np.full(ny * nz, 0.05, np.float32)
+    fields = wl.uniform_flow(nx * ny * nz, 0.05)
+    o = oracle.Oracle(1, nx, ny, nz, 0.58)
+    o.set_flags(flags)
+    o.set_inlet(inlet)
+    o.reinit(*fields)
+    o.step(6)
+    ref = o.state()
+    for variant in (0, 1, 2, 3, -1):
+        with fs.Solver(1, nx, ny, nz, 0.58) as s:
+            s.set_flags(flags)
+            s.set_inlet_profile(inlet)
+            s.set_variant(variant)
+            s.reinit_from_macro(*fields)
+            if variant == -1:
+                before = s.state()
+                chosen = s.autotune(3)
+                assert chosen in (0, 1, 2, 3) and s.variant == chosen
+                np.testing.assert_array_equal(s.state(), before)
+            s.step(6)
+            np.testing.assert_array_equal(s.state(), ref)
