@@ -198,6 +198,13 @@ def main():
               f"{r['achieved_GBps']:.0f} GB/s", file=sys.stderr, flush=True)
     ceiling = copy_ceiling()
     head = methods[runs[0][0]]
+    # HBM bytes per BGK launch from the committed rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE runs of this workload (calibrated against the copy kernel)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_channel384.json")
+    if args.workload == "channel" and os.path.exists(tpath):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get("bgk_bytes_per_launch")
     line = {
         "metric": "MLUPS & % HBM roofline (BGK/HOME/hybrid); batched & adjoint MLUPS vs plain",
         "value": head["MLUPS"],
@@ -214,7 +221,8 @@ def main():
         "config": {"workload": desc, "cells": case.n, "fluid_cells": case.fluid()},
         "methods": methods,
         "roofline": {"bound": "hbm", "achieved": head["achieved_GBps"], "peak": PEAK_HBM_GBPS,
-                     "unit": "GB/s", "frac": head["roofline_frac"], "traffic": None,
+                     "unit": "GB/s", "frac": head["roofline_frac"], "traffic": traffic,
+                     "algorithmic_bytes_per_launch": round(case.fluid() * B_LUP[runs[0][1]]),
                      "kernel": "k_pop_step (BGK, AA pattern)",
                      "measured_copy_ceiling_GBps": round(ceiling, 1)},
     }
