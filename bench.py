@@ -151,12 +151,90 @@ def cpu_baseline(case, budget_s=12.0):
                       f"(FP32 oracle, OpenMP, {dt:.1f} s)"}
 
 
+def batch_bench(steps, warmup, seed=7):
+    """The batched sweep: 64 seeded domains (edges 32..128, tau 0.52..0.9,
+    mixed BGK / HOME / Hybrid, half Taylor-Green, half icosphere channels)
+    stepped together in one batch, against the same 64 runs one after the
+    other on single-domain solvers -- both with a synchronising step(1) per
+    step (how the reference's run loops call step) and with step(K) queued
+    asynchronously."""
+    from myfluidsimulation_amd import Batch, Solver
+    from myfluidsimulation_amd import workloads as wl
+
+    specs = wl.batch_sweep_specs(64, seed)
+    with tempfile.TemporaryDirectory() as td:
+        built = [wl.build_sweep_domain(sp, td) for sp in specs]
+    cells = sum(sp["nx"] * sp["ny"] * sp["nz"] for sp in specs)
+
+    def setup(target, i):
+        flags, fields, inlet = built[i]
+        if isinstance(target, Batch):
+            if flags is not None:
+                target.set_flags(i, flags)
+            if inlet is not None:
+                target.set_inlet_profile(i, inlet)
+            target.reinit_from_macro(i, *fields)
+        else:
+            if flags is not None:
+                target.set_flags(flags)
+            if inlet is not None:
+                target.set_inlet_profile(inlet)
+            target.reinit_from_macro(*fields)
+
+    b = Batch([(sp["method"], sp["nx"], sp["ny"], sp["nz"], sp["tau"]) for sp in specs])
+    for i in range(len(specs)):
+        setup(b, i)
+    tuned = b.autotune(10)
+    if warmup:
+        b.step(warmup + (warmup & 1))
+    t0 = time.perf_counter()
+    ms = b.timed_steps(steps)
+    wall_b = time.perf_counter() - t0
+    finite = all(bool(np.isfinite(b.macro(i, 0)).all()) for i in range(len(specs)))
+    b.close()
+    seq_sync = seq_async = 0.0
+    seq_events = 0.0
+    for i, sp in enumerate(specs):
+        s = Solver(sp["method"], sp["nx"], sp["ny"], sp["nz"], sp["tau"])
+        setup(s, i)
+        s.step(2)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s.step(1)
+        seq_sync += time.perf_counter() - t0
+        t0 = time.perf_counter()
+        seq_events += s.timed_steps(steps) * 1e-3
+        seq_async += time.perf_counter() - t0
+        s.close()
+    lup = cells * steps
+    res = {
+        "metric": "batched MLUPS vs sequential single-domain runs (64-domain sweep)",
+        "value": round(lup / (ms * 1e-3) / 1e6, 1), "unit": "MLUPS", "n_gpus": 1,
+        "steps": steps, "warmup": warmup, "ms_per_step": round(ms / steps, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded)",
+        "config": {"workload": f"64 domains, edges 32..128, seed {seed}, {cells} cells, "
+                               "mixed BGK/HOME/Hybrid, half TGV, half icosphere channels",
+                   "cells": cells, "autotuned_grid": {"pop_blocks": tuned[0],
+                                                      "home_blocks": tuned[1]}},
+        "batched_wall_MLUPS": round(lup / wall_b / 1e6, 1),
+        "sequential_sync_step1_MLUPS": round(lup / seq_sync / 1e6, 1),
+        "sequential_async_MLUPS": round(lup / seq_async / 1e6, 1),
+        "sequential_async_event_MLUPS": round(lup / seq_events / 1e6, 1),
+        "speedup_vs_sync_step1": round(seq_sync / (ms * 1e-3), 2),
+        "speedup_vs_async": round(seq_async / (ms * 1e-3), 2),
+        "finite": finite,
+    }
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="channel", choices=["channel", "tgv512", "channel768"])
+    ap.add_argument("--workload", default="channel",
+                    choices=["channel", "tgv512", "channel768", "batch64"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -171,6 +249,10 @@ def main():
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
 
+    if args.workload == "batch64":
+        print(json.dumps(batch_bench(args.steps or 2000, 20 if args.warmup is None
+                                     else args.warmup)))
+        return
     if args.workload == "channel":
         case = channel_case(384, 128, 128, (96, 64, 64), 20.0)
         runs = [("bgk", 0, 0), ("home", 1, 0), ("hybrid", 2, 2)]

@@ -166,6 +166,91 @@ int fs_solver_state_size(const fs_solver* s, size_t* nfloats);
 int fs_solver_copy_state(fs_solver* s, float* host_dst);
 int fs_solver_load_state(fs_solver* s, const float* host_src);
 
+/* ===================== batched domains ================================== */
+/* Many independent lattices of uneven size stepped together (replaces the
+ * reference's one-process-per-configuration sweeps, main_compare.cpp:511-699,
+ * scripts/run_b0_sweeps.ps1).  Every domain gives the same bits as a
+ * single-domain solver with the same inputs.  Domains share one step
+ * counter; the step loop replays a captured hipGraph. */
+typedef struct fs_domain_desc {
+  int method; /* FS_METHOD_* */
+  int nx, ny, nz;
+  float tau;
+  float fx, fy, fz;
+} fs_domain_desc;
+typedef struct fs_batch fs_batch;
+
+/* Allocates and resets (rho=1, u=0) every domain. */
+int fs_batch_create(const fs_domain_desc* descs, int ndomains, fs_batch** out);
+int fs_batch_destroy(fs_batch* b);
+int fs_batch_info(const fs_batch* b, int* ndomains, size_t* cells);
+/* Per-domain set-up, as fs_solver_set_flags / _set_inlet_profile /
+ * _reinit_from_macro.  A HOME domain's outlet cell may not sit at an x that
+ * is a multiple of 32 (FS_ERR_INVALID).  Re-initialisation needs an even
+ * number of batch steps since creation (FS_ERR_STATE otherwise). */
+int fs_batch_set_flags(fs_batch* b, int domain, const uint8_t* host_flags);
+int fs_batch_set_inlet_profile(fs_batch* b, int domain, const float* host_ux);
+int fs_batch_reinit_from_macro(fs_batch* b, int domain, const float* d_rho,
+                               const float* d_ux, const float* d_uy,
+                               const float* d_uz);
+/* nsteps for every domain; synchronous.  timed_steps: HIP events on the
+ * batch's stream around the steps, elapsed device milliseconds. */
+int fs_batch_step(fs_batch* b, int nsteps);
+int fs_batch_timed_steps(fs_batch* b, int nsteps, float* elapsed_ms);
+/* Launch shape: workgroups of the populations and HOME launches (0 = one per
+ * tile, otherwise a persistent grid that strides over the tile list) and
+ * whether the step loop replays a captured hipGraph. */
+int fs_batch_set_launch(fs_batch* b, int pop_blocks, int home_blocks, int use_graph);
+int fs_batch_get_launch(const fs_batch* b, int* pop_blocks, int* home_blocks, int* use_graph);
+/* Times the candidate grid sizes (trial_steps, even, each from the current
+ * state, which is restored) and keeps the fastest pair. */
+int fs_batch_autotune(fs_batch* b, int trial_steps, int* pop_blocks, int* home_blocks);
+/* Results of one domain, as fs_solver_copy_macro / fs_solver_copy_state. */
+int fs_batch_copy_macro(fs_batch* b, int domain, int which, float* host_dst);
+int fs_batch_copy_state(fs_batch* b, int domain, float* host_dst);
+
+/* ===================== differentiable runs ============================== */
+/* Extension with no counterpart in the reference (its solvers have no
+ * adjoint): reverse-mode gradients of a T-step run of a solver through
+ * hand-written adjoint kernels, with sqrt(T) checkpointing and segment-wise
+ * recomputation of the forward trajectory.  Inputs are the initial macro
+ * fields (rho0, u0) and the inlet profile; outputs are the solver's macro
+ * fields after T steps (as fs_solver_macro_ptr reports them) and the
+ * x-momentum D^n handed to "body" solid cells in each step n by halfway
+ * bounce-back (momentum exchange, 2 e_qx f*_q summed over the body links).
+ * All pointers are device pointers; work is enqueued on the solver's stream
+ * (fs_solver_set_stream) and these calls return without waiting.  The
+ * solver's flags, parameters and variant must not change while the handle
+ * lives.  Inlet cells, if any, must sit at x = 0. */
+typedef struct fs_adjoint fs_adjoint;
+
+/* segment: steps between checkpoints (0 = ceil(sqrt(T))).  host_body: N
+ * host bytes, nonzero marks the solid cells whose links count toward the
+ * drag, or NULL for every solid cell. */
+int fs_adjoint_create(fs_solver* s, int T, int segment, const uint8_t* host_body,
+                      fs_adjoint** out);
+int fs_adjoint_destroy(fs_adjoint* a);
+int fs_adjoint_info(const fs_adjoint* a, int* T, int* segment, int* checkpoints,
+                    size_t* device_bytes);
+/* Re-initialises the solver at equilibrium from (rho0, u0) (N floats each;
+ * NULL: rho 1, u 0), sets the inlet profile (ny*nz floats, NULL keeps the
+ * solver's), runs T steps storing the checkpoints, copies the end macro
+ * fields to out_* (each may be NULL) and D^0..D^{T-1} to drag (T floats,
+ * may be NULL). */
+int fs_adjoint_forward(fs_adjoint* a, const float* rho0, const float* ux0,
+                       const float* uy0, const float* uz0, const float* inlet,
+                       float* out_rho, float* out_ux, float* out_uy,
+                       float* out_uz, float* drag);
+/* Given dL/d(end macro fields) (N floats each) and dL/dD (T floats), any of
+ * them NULL for zero, writes dL/d(rho0, u0) (N floats each) and dL/d(inlet)
+ * (ny*nz floats), any of them NULL to skip, for the last forward pass.
+ * Leaves the solver at its end state; may be called again with other
+ * gradients.  FS_ERR_STATE before a forward pass. */
+int fs_adjoint_backward(fs_adjoint* a, const float* g_rho, const float* g_ux,
+                        const float* g_uy, const float* g_uz, const float* g_drag,
+                        float* grad_rho0, float* grad_ux0, float* grad_uy0,
+                        float* grad_uz0, float* grad_inlet);
+
 /* ===================== geometry ========================================= */
 
 /* load_stl(path, TriangleMesh&) (stl_loader.cpp:102-122): triangle soup,

@@ -73,6 +73,24 @@ SIGNATURES = {
     "fs_solver_state_size": (_i, [_p, C.POINTER(_sz)]),
     "fs_solver_copy_state": (_i, [_p, _p]),
     "fs_solver_load_state": (_i, [_p, _p]),
+    "fs_batch_create": (_i, [_p, _i, C.POINTER(_p)]),
+    "fs_batch_destroy": (_i, [_p]),
+    "fs_batch_info": (_i, [_p, C.POINTER(_i), C.POINTER(_sz)]),
+    "fs_batch_set_flags": (_i, [_p, _i, _p]),
+    "fs_batch_set_inlet_profile": (_i, [_p, _i, _p]),
+    "fs_batch_reinit_from_macro": (_i, [_p, _i, _p, _p, _p, _p]),
+    "fs_batch_step": (_i, [_p, _i]),
+    "fs_batch_timed_steps": (_i, [_p, _i, C.POINTER(_f)]),
+    "fs_batch_set_launch": (_i, [_p, _i, _i, _i]),
+    "fs_batch_get_launch": (_i, [_p, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i)]),
+    "fs_batch_autotune": (_i, [_p, _i, C.POINTER(_i), C.POINTER(_i)]),
+    "fs_batch_copy_macro": (_i, [_p, _i, _i, _p]),
+    "fs_batch_copy_state": (_i, [_p, _i, _p]),
+    "fs_adjoint_create": (_i, [_p, _i, _i, _p, C.POINTER(_p)]),
+    "fs_adjoint_destroy": (_i, [_p]),
+    "fs_adjoint_info": (_i, [_p, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_sz)]),
+    "fs_adjoint_forward": (_i, [_p] + [_p] * 10),
+    "fs_adjoint_backward": (_i, [_p] + [_p] * 10),
     "fs_load_stl": (_i, [C.c_char_p, _p, _i, C.POINTER(_i), _p, _p]),
     "fs_voxelize": (_i, [_p, _i, _p, _p, _i, _i, _i, _f, _f, _f, _f, _i, _f, _p]),
     "fs_band": (_i, [_p, _i, _i, _i, _i, _p]),
@@ -96,6 +114,14 @@ SIGNATURES = {
     "reinit_equilibrium_from_macro": (None, [_p, _p, _p, _p, _p, _i]),
     "launch_speed": (None, [_p, _p, _p, _p, _i]),
 }
+
+
+
+class DomainDesc(C.Structure):
+    """fs_domain_desc"""
+    _fields_ = [("method", C.c_int), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("tau", C.c_float), ("fx", C.c_float), ("fy", C.c_float), ("fz", C.c_float)]
+
 
 _LIB = None
 

@@ -17,8 +17,9 @@
 #include <string>
 #include <vector>
 
-#include "../../include/fluidsim.h"
+#include "fluidsim.h"
 #include "fs_internal.hpp"
+#include "solver_impl.hpp"
 
 namespace fs {
 bool read_stl(const char* path, std::vector<float>& xyz);
@@ -28,10 +29,10 @@ bool vtk_vector(const char* path, const float* a, const float* b, const float* c
 }  // namespace fs
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
 
-int fail(int code, const char* fmt, ...) {
+int fs::fail(int code, const char* fmt, ...) {
   char buf[1024];
   va_list ap;
   va_start(ap, fmt);
@@ -41,11 +42,8 @@ int fail(int code, const char* fmt, ...) {
   return code;
 }
 
-#define FS_HIP(expr)                                                                   \
-  do {                                                                                 \
-    hipError_t e_ = (expr);                                                            \
-    if (e_ != hipSuccess) return fail(FS_ERR_DEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
-  } while (0)
+namespace {
+using fs::fail;
 
 // Runs `body` and maps escaping C++ exceptions to status codes.
 template <class Fn>
@@ -67,31 +65,6 @@ bool fault_injected(const char* what) {
 }
 
 }  // namespace
-
-struct fs_solver {
-  int method = 0;
-  fs::Geo g{};
-  fs::Phys ph{};
-  float tau = 0.6f;
-  hipStream_t own = nullptr;
-  hipStream_t stream = nullptr;
-  // populations (BGK / Hybrid)
-  float* F = nullptr;
-  float* macro = nullptr;  // 4 planes: rho, ux, uy, uz
-  int parity = 0;          // AA representation: 0 natural, 1 swapped
-  // moments (HOME)
-  float* mbuf[2] = {nullptr, nullptr};
-  int cur = 0;
-  int home_variant = -1;  // HOME kernel variant, -1 = default for the grid
-  // flags
-  uint8_t* flags = nullptr;  // derived flag byte used by the kernels
-  uint8_t* stage = nullptr;  // public flag bytes uploaded from the host
-  std::vector<uint8_t> hflags;
-  float* inlet = nullptr;  // ny*nz
-  float* scratch = nullptr;  // 19 planes for state conversion (lazy)
-
-  float* mom_cur() const { return mbuf[cur]; }
-};
 
 namespace {
 
@@ -115,17 +88,6 @@ void free_solver(fs_solver* s) {
   delete s;
 }
 
-fs::PopLaunch pop_args(fs_solver* s) {
-  fs::PopLaunch a;
-  a.F = s->F;
-  a.flags = s->flags;
-  a.rho = s->macro;
-  a.ux = s->macro + s->g.P;
-  a.uy = s->macro + 2 * s->g.P;
-  a.uz = s->macro + 3 * s->g.P;
-  a.inlet = s->inlet;
-  return a;
-}
 
 int upload_flags(fs_solver* s) {
   FS_HIP(hipMemcpyAsync(s->stage, s->hflags.data(), s->g.n, hipMemcpyHostToDevice, s->stream));
@@ -168,6 +130,43 @@ int reset_impl(fs_solver* s) {
   FS_HIP(hipStreamSynchronize(s->stream));
   return FS_OK;
 }
+
+}  // namespace
+
+fs::PopLaunch pop_args(fs_solver* s) {
+  fs::PopLaunch a;
+  a.F = s->F;
+  a.flags = s->flags;
+  a.rho = s->macro;
+  a.ux = s->macro + s->g.P;
+  a.uy = s->macro + 2 * s->g.P;
+  a.uz = s->macro + 3 * s->g.P;
+  a.inlet = s->inlet;
+  return a;
+}
+
+int solver_reinit_async(fs_solver* s, const float* rho, const float* ux, const float* uy,
+                        const float* uz) {
+  if (s->method == fs::kMethodHOME) {
+    // kern_reinit_from_macro (lbm3d_home.cu:109-137) into both buffers
+    FS_HIP(fs::launch_home_fill(s->g, s->mbuf[0], s->mbuf[1], rho, ux, uy, uz, s->stream));
+    s->cur = 0;
+  } else if (s->method == fs::kMethodBGK) {
+    // a null rho means reset() (lbm3d_legacy.cu:174-178)
+    if (!rho) {
+      FS_HIP(fs::launch_pop_reset(s->g, pop_args(s), s->stream));
+    } else {
+      FS_HIP(fs::launch_pop_reinit(s->g, pop_args(s), 0, rho, ux, uy, uz, s->stream));
+    }
+    s->parity = 0;
+  } else {
+    FS_HIP(fs::launch_pop_reinit(s->g, pop_args(s), 1, rho, ux, uy, uz, s->stream));
+    s->parity = 0;
+  }
+  return FS_OK;
+}
+
+namespace {
 
 int enqueue_steps(fs_solver* s, int nsteps) {
   if (nsteps < 0) return fail(FS_ERR_INVALID, "nsteps must be >= 0");
@@ -379,19 +378,7 @@ int fs_solver_reset(fs_solver* s) {
 int fs_solver_reinit_from_macro(fs_solver* s, const float* rho, const float* ux, const float* uy,
                                 const float* uz) {
   if (int rc = check_solver(s)) return rc;
-  if (s->method == fs::kMethodHOME) {
-    // kern_reinit_from_macro (lbm3d_home.cu:109-137) into both buffers
-    FS_HIP(fs::launch_home_fill(s->g, s->mbuf[0], s->mbuf[1], rho, ux, uy, uz, s->stream));
-    s->cur = 0;
-  } else if (s->method == fs::kMethodBGK) {
-    // a null rho means reset() (lbm3d_legacy.cu:174-178)
-    if (!rho) return reset_impl(s);
-    FS_HIP(fs::launch_pop_reinit(s->g, pop_args(s), 0, rho, ux, uy, uz, s->stream));
-    s->parity = 0;
-  } else {
-    FS_HIP(fs::launch_pop_reinit(s->g, pop_args(s), 1, rho, ux, uy, uz, s->stream));
-    s->parity = 0;
-  }
+  if (int rc = solver_reinit_async(s, rho, ux, uy, uz)) return rc;
   FS_HIP(hipStreamSynchronize(s->stream));
   return FS_OK;
 }

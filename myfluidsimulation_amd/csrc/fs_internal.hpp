@@ -38,6 +38,7 @@ constexpr uint8_t F_INLET = 0x04;
 constexpr uint8_t F_OUTLET = 0x08;
 constexpr uint8_t F_NEAR = 0x10;    // one of the 18 periodic neighbours is solid
 constexpr uint8_t F_FEEDER = 0x20;  // cell x+1 is an outlet: emit for it too
+constexpr uint8_t F_BODY = 0x40;    // adjoint flags only: a solid whose links count for the drag
 constexpr uint8_t F_PUBLIC = 0x0F;
 
 constexpr int kMethodBGK = 0;
@@ -144,6 +145,88 @@ hipError_t launch_rho_slab(float* rho, int nx, int ny, int nz, int axis, float a
                            float width, hipStream_t st);
 hipError_t launch_reinit_cs2(const float* rho, const float* ux, const float* uy, const float* uz,
                              float* f, size_t n, size_t stride, hipStream_t st);
+
+// batched domains (batch.hip): descriptor table entries
+struct BatchPopDomain {
+  Geo g;
+  Phys ph;
+  PopLaunch a;
+  int hybrid;
+  uint32_t tile0;          // first work-list tile of this domain
+  uint32_t tilesPerPlane;  // 256-cell tiles per z-plane
+};
+struct BatchHomeDomain {
+  Geo g;
+  Phys ph;
+  float* m0;
+  float* m1;
+  const uint8_t* flags;
+  const float* inlet;
+  uint32_t tile0;
+  int tilesX, chunks;
+};
+void batch_home_tiling(const Geo& g, int& tilesX, int& tilesY, int& chunks);
+int batch_home_tile_width();
+// blocks <= 0: one workgroup per tile; otherwise a persistent grid of that size
+hipError_t launch_batch_pop(const BatchPopDomain* doms, const uint16_t* tile_dom, uint32_t ntiles,
+                            bool odd, bool macro, int blocks, hipStream_t st);
+hipError_t launch_batch_home(const BatchHomeDomain* doms, const uint16_t* tile_dom,
+                             uint32_t ntiles, int cur, int blocks, hipStream_t st);
+
+// ---- adjoint (adjoint.hip) ----
+// Backward through one population step n (BGK / Hybrid), canonical layout:
+// lin = dL/df^{n+1}, lout = dL/df^n (19 planes each).
+struct AdjPopLaunch {
+  const float* lin;
+  float* lout;
+  const float *rho, *ux, *uy, *uz;      // the step's collision moments (u with the shift F)
+  const uint8_t* flags;                 // derived flags with F_BODY
+  const float* inlet;                   // ny*nz inlet x-velocity
+  const float *grho, *gux, *guy, *guz;  // dL/d(macros written by this step), or null
+  const float* gdrag;                   // dL/dD^n (one device float), or null
+  float* ginlet;                        // dL/d inlet accumulator (ny*nz), or null
+};
+hipError_t launch_adj_pop(const Geo& g, const Phys& ph, const AdjPopLaunch& a, bool hybrid,
+                          hipStream_t st);
+// dL/d(rho0, u0) of the equilibrium initial populations from dL/df^0
+hipError_t launch_adj_pop_init(const Geo& g, const float* lam, const float* rho0, const float* ux0,
+                               const float* uy0, const float* uz0, float* grho, float* gux,
+                               float* guy, float* guz, hipStream_t st);
+// Backward through one HOME step n: lin = dL/dR^{n+1} (10 planes: raw moments
+// r, j, M of the gathered populations), m = m^n; writes lout = dL/dR^n, or,
+// when final (n = 0), dL/dm^0 components rho, u into grho..guz.
+struct AdjHomeLaunch {
+  const float* lin;
+  float* lout;
+  const float* m;
+  const uint8_t* flags;
+  const float* gdrag;
+  float* ginlet;
+  float *grho, *gux, *guy, *guz;
+};
+hipError_t launch_adj_home(const Geo& g, const Phys& ph, const AdjHomeLaunch& a, bool final_step,
+                           hipStream_t st);
+// dL/dR^T from dL/d(end moments rho, u) (nullable planes) and m^T
+hipError_t launch_adj_home_seed(const Geo& g, const Phys& ph, const float* m, const uint8_t* flags,
+                                const float* grho, const float* gux, const float* guy,
+                                const float* guz, float* lout, float* ginlet, hipStream_t st);
+// x-momentum handed to body cells in one step, summed over the link list
+// (cell, direction q with e_qx != 0, neighbour along q a body cell):
+// populations: 2 e_qx f*_q read from the post-step AA slot F[OPP q][cell];
+// moments: 2 e_qx f*_q rebuilt from m^n.  Deterministic two-pass sum.
+struct DragLink {
+  uint32_t cell;
+  uint32_t q;
+};
+hipError_t launch_drag_pop(const Geo& g, const float* F, const DragLink* links, uint32_t nlinks,
+                           float* partials, int nblocks, float* out, hipStream_t st);
+hipError_t launch_drag_home(const Geo& g, const Phys& ph, const float* m, const DragLink* links,
+                            uint32_t nlinks, float* partials, int nblocks, float* out,
+                            hipStream_t st);
+hipError_t launch_fill(float* p, float v, size_t n, hipStream_t st);
+
+// error reporting shared by the C ABI sources (capi.cpp, batch_api.cpp)
+int fail(int code, const char* fmt, ...);
 
 // plain streaming copy (bandwidth ceiling probe)
 hipError_t launch_copy_probe(void* dst, const void* src, size_t bytes, hipStream_t st);

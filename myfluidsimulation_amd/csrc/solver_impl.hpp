@@ -1,0 +1,49 @@
+// solver_impl.hpp -- the solver handle behind the C ABI, shared by the ABI
+// sources (capi.cpp owns it; adjoint_api.cpp drives its kernels).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "fluidsim.h"
+#include "fs_internal.hpp"
+
+// Maps a failing HIP call to FS_ERR_DEVICE with the call's text.
+#define FS_HIP(expr)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess) return fs::fail(FS_ERR_DEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+struct fs_solver {
+  int method = 0;
+  fs::Geo g{};
+  fs::Phys ph{};
+  float tau = 0.6f;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;
+  // populations (BGK / Hybrid)
+  float* F = nullptr;
+  float* macro = nullptr;  // 4 planes: rho, ux, uy, uz
+  int parity = 0;          // AA representation: 0 natural, 1 swapped
+  // moments (HOME)
+  float* mbuf[2] = {nullptr, nullptr};
+  int cur = 0;
+  int home_variant = -1;  // HOME kernel variant, -1 = default for the grid
+  // flags
+  uint8_t* flags = nullptr;  // derived flag byte used by the kernels
+  uint8_t* stage = nullptr;  // public flag bytes uploaded from the host
+  std::vector<uint8_t> hflags;
+  float* inlet = nullptr;  // ny*nz
+  float* scratch = nullptr;  // 19 planes for state conversion (lazy)
+
+  float* mom_cur() const { return mbuf[cur]; }
+};
+
+// Kernel arguments of the population step on the solver's own buffers.
+fs::PopLaunch pop_args(fs_solver* s);
+// Re-initialises the solver's state from macro fields (device pointers,
+// nullable) without synchronising the stream.
+int solver_reinit_async(fs_solver* s, const float* rho, const float* ux, const float* uy,
+                        const float* uz);

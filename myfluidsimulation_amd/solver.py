@@ -250,5 +250,114 @@ def band(solid, nx, ny, nz, d0) -> np.ndarray:
     return out
 
 
-__all__ = ["Solver", "DeviceBuffer", "load_stl", "voxelize", "band", "BGK", "HOME", "HYBRID",
-           "capi"]
+__all__ = ["Solver", "Batch", "DeviceBuffer", "load_stl", "voxelize", "band", "BGK", "HOME",
+           "HYBRID", "capi"]
+
+
+class Batch:
+    """Many independent domains stepped together (fs_batch_*).
+
+    domains: list of (method, nx, ny, nz, tau[, (fx, fy, fz)]) tuples."""
+
+    def __init__(self, domains):
+        descs = (capi.DomainDesc * len(domains))()
+        self.dims = []
+        self.methods = []
+        for i, d in enumerate(domains):
+            m, nx, ny, nz, tau = d[:5]
+            f = d[5] if len(d) > 5 else (0.0, 0.0, 0.0)
+            descs[i] = capi.DomainDesc(int(m), int(nx), int(ny), int(nz), float(tau), *map(float, f))
+            self.dims.append((int(nx), int(ny), int(nz)))
+            self.methods.append(int(m))
+        h = C.c_void_p()
+        call("fs_batch_create", C.cast(descs, C.c_void_p), len(domains), C.byref(h))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().fs_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __len__(self):
+        return len(self.dims)
+
+    def cells(self) -> int:
+        n = C.c_size_t()
+        call("fs_batch_info", self.h, None, C.byref(n))
+        return n.value
+
+    def _n(self, d):
+        nx, ny, nz = self.dims[d]
+        return nx * ny * nz
+
+    def set_flags(self, d, flags):
+        a = np.ascontiguousarray(np.asarray(flags, np.uint8).reshape(-1))
+        assert a.size == self._n(d)
+        call("fs_batch_set_flags", self.h, int(d), _host_ptr(a))
+
+    def set_inlet_profile(self, d, ux=None):
+        if ux is None:
+            call("fs_batch_set_inlet_profile", self.h, int(d), None)
+        else:
+            a = np.ascontiguousarray(np.asarray(ux, np.float32).reshape(-1))
+            call("fs_batch_set_inlet_profile", self.h, int(d), _host_ptr(a))
+
+    def reinit_from_macro(self, d, rho=None, ux=None, uy=None, uz=None):
+        keep = []
+
+        def dev(a):
+            if a is None:
+                return None
+            b = DeviceBuffer.from_host(np.asarray(a, np.float32).reshape(-1))
+            keep.append(b)
+            return C.c_void_p(b.ptr)
+
+        call("fs_batch_reinit_from_macro", self.h, int(d), dev(rho), dev(ux), dev(uy), dev(uz))
+        for b in keep:
+            b.free()
+
+    def step(self, nsteps=1):
+        call("fs_batch_step", self.h, int(nsteps))
+
+    def timed_steps(self, nsteps) -> float:
+        ms = C.c_float()
+        call("fs_batch_timed_steps", self.h, int(nsteps), C.byref(ms))
+        return float(ms.value)
+
+    def set_launch(self, pop_blocks=0, home_blocks=0, use_graph=True):
+        call("fs_batch_set_launch", self.h, int(pop_blocks), int(home_blocks), int(bool(use_graph)))
+
+    def launch(self):
+        p, h, g = C.c_int(), C.c_int(), C.c_int()
+        call("fs_batch_get_launch", self.h, C.byref(p), C.byref(h), C.byref(g))
+        return p.value, h.value, bool(g.value)
+
+    def autotune(self, trial_steps=10):
+        p, h = C.c_int(), C.c_int()
+        call("fs_batch_autotune", self.h, int(trial_steps), C.byref(p), C.byref(h))
+        return p.value, h.value
+
+    def macro(self, d, which) -> np.ndarray:
+        out = np.empty(self._n(d), np.float32)
+        call("fs_batch_copy_macro", self.h, int(d), int(which), _host_ptr(out))
+        return out
+
+    def macros(self, d):
+        return [self.macro(d, k) for k in range(4)]
+
+    def state(self, d) -> np.ndarray:
+        out = np.empty(self._n(d) * (10 if self.methods[d] == HOME else 19), np.float32)
+        call("fs_batch_copy_state", self.h, int(d), _host_ptr(out))
+        return out

@@ -68,6 +68,34 @@ def kinetic_energy(rho, ux, uy, uz) -> float:
     return float(0.5 * np.mean(u * u + v * v + w * w))
 
 
+def fourier_modes(nx, ny, nz, n_modes=8, amp=0.01, seed=1):
+    """Seeded velocity perturbation: a sum of n_modes periodic Fourier modes
+    amp * d_m * sin(k_m . x + phi_m) with integer wave numbers k_m in
+    [-2, 2]^3 (not all zero), phase phi_m ~ U[0, 2pi) and a unit direction
+    d_m perpendicular to k_m (so each mode is divergence-free), at cell
+    centres.  Returns float32 (ux, uy, uz) of nx*ny*nz, x fastest."""
+    rng = SplitMix64(seed)
+    z, y, x = np.meshgrid(np.arange(nz) + 0.5, np.arange(ny) + 0.5, np.arange(nx) + 0.5,
+                          indexing="ij")
+    u = [np.zeros_like(x) for _ in range(3)]
+    for _ in range(n_modes):
+        k = [0, 0, 0]
+        while k == [0, 0, 0]:
+            k = [min(4, int(rng.uniform() * 5)) - 2 for _ in range(3)]
+        phi = 2 * np.pi * rng.uniform()
+        r = np.array([2 * rng.uniform() - 1 for _ in range(3)])
+        kv = np.array(k, float)
+        d = r - kv * (r @ kv) / (kv @ kv)
+        if np.linalg.norm(d) < 1e-6:
+            d = np.cross(kv, [1.0, 0.0, 0.0] if abs(kv[0]) < abs(kv).max() else [0.0, 1.0, 0.0])
+        d = d / np.linalg.norm(d)
+        ph = 2 * np.pi * (k[0] * x / nx + k[1] * y / ny + k[2] * z / nz) + phi
+        s = amp * np.sin(ph)
+        for a in range(3):
+            u[a] = u[a] + d[a] * s
+    return tuple(np.ascontiguousarray(c.reshape(-1), dtype=np.float32) for c in u)
+
+
 # ---------------------------------------------------------------- icosphere
 def icosphere(subdiv: int = 3):
     """Unit icosphere: (vertices (V,3) float64, faces (F,3) int), outward
@@ -169,3 +197,56 @@ def uniform_flow(n, u_in):
     ux = np.full(n, u_in, np.float32)
     z = np.zeros(n, np.float32)
     return rho, ux, z, z.copy()
+
+
+# ---------------------------------------------------------------- batched sweep
+SWEEP_EDGES = tuple(range(32, 129, 16))  # 32, 48, ..., 128
+
+
+def batch_sweep_specs(n_domains=64, seed=7, channel_u=0.04):
+    """The batched B0 / tau sweep: each domain draws its three edges from
+    {32, 48, ..., 128}, tau ~ U[0.52, 0.9], a method from {BGK, HOME, Hybrid}
+    and a Hybrid band from {1, 2, 3}; the first half are Taylor-Green
+    vortices (u0 ~ U[0.02, 0.08]), the second half channels with a seeded
+    jittered icosphere of radius 0.15 * (smallest edge)."""
+    rng = SplitMix64(seed)
+    pick = lambda seq: seq[min(len(seq) - 1, int(rng.uniform() * len(seq)))]  # noqa: E731
+    specs = []
+    for i in range(n_domains):
+        nx, ny, nz = pick(SWEEP_EDGES), pick(SWEEP_EDGES), pick(SWEEP_EDGES)
+        tau = 0.52 + 0.38 * rng.uniform()
+        method = pick((0, 1, 2))
+        band = pick((1, 2, 3))
+        spec = dict(index=i, method=method, nx=nx, ny=ny, nz=nz, tau=tau, band=band)
+        if i < n_domains // 2:
+            spec.update(kind="tgv", u0=0.02 + 0.06 * rng.uniform())
+        else:
+            spec.update(kind="channel", u_in=channel_u,
+                        radius=0.15 * min(nx, ny, nz), center=(nx / 4.0, ny / 2.0, nz / 2.0),
+                        stl_seed=seed * 1000 + i)
+        specs.append(spec)
+    return specs
+
+
+def build_sweep_domain(spec, tmpdir):
+    """Flags (or None), initial fields and inlet profile (or None) of one
+    sweep domain; the channel obstacle is written as STL and voxelized on the
+    GPU.  Returns (flags, fields, inlet)."""
+    import os
+
+    from . import solver as S
+
+    nx, ny, nz = spec["nx"], spec["ny"], spec["nz"]
+    if spec["kind"] == "tgv":
+        return None, taylor_green_3d(nx, ny, nz, spec["u0"]), None
+    path = os.path.join(tmpdir, f"sweep_{spec['index']}.stl")
+    scale, trans = obstacle_stl(path, nx, ny, nz, spec["center"], spec["radius"],
+                                spec["stl_seed"])
+    tris, bmin, bmax = S.load_stl(path)
+    solid = S.voxelize(tris, bmin, bmax, nx, ny, nz, scale, trans)
+    flags = channel_flags(nx, ny, nz, solid)
+    if spec["method"] == 2:
+        band = S.band((flags & FLAG_SOLID).astype(np.uint8), nx, ny, nz, spec["band"])
+        flags = flags | (band * 2).astype(np.uint8)
+    n = nx * ny * nz
+    return flags, uniform_flow(n, spec["u_in"]), np.full(ny * nz, spec["u_in"], np.float32)

@@ -1,0 +1,117 @@
+"""GPU: the batched engine against single-domain solvers (which are checked
+bit for bit against the oracle elsewhere) and directly against the oracle."""
+import numpy as np
+import pytest
+
+from myfluidsimulation_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def small_domains():
+    """Mixed methods, uneven sizes, periodic and channel domains."""
+    out = []
+    rng = np.random.default_rng(11)
+    shapes = [(32, 48, 16), (48, 32, 32), (80, 16, 18), (64, 20, 24), (34, 18, 40), (96, 32, 8),
+              (33, 17, 9)]
+    for i, (nx, ny, nz) in enumerate(shapes):
+        method = i % 3
+        tau = 0.55 + 0.3 * rng.random()
+        n = nx * ny * nz
+        if i % 2 == 0:
+            fields = wl.taylor_green_3d(nx, ny, nz, 0.03 + 0.01 * i)
+            solid = np.zeros((nz, ny, nx), np.uint8)
+            solid[nz // 3: nz // 2, ny // 3: ny // 2, nx // 4: nx // 2] = 1
+            flags = solid.reshape(-1)
+            inlet = None
+        else:
+            obst = np.zeros((nz, ny, nx), np.uint8)
+            obst[nz // 3: nz // 2 + 1, ny // 3: ny // 2 + 1, nx // 4: nx // 3 + 1] = 1
+            flags = wl.channel_flags(nx, ny, nz, obst.reshape(-1))
+            fields = wl.uniform_flow(n, 0.04)
+            inlet = np.full(ny * nz, 0.04, np.float32)
+        if method == 2:
+            import pyoracle
+
+            band = pyoracle.band((flags & 1).astype(np.uint8), nx, ny, nz, 1 + i % 3)
+            flags = flags | (band * 2).astype(np.uint8)
+        out.append(dict(method=method, dims=(nx, ny, nz), tau=tau, flags=flags, fields=fields,
+                        inlet=inlet, force=(1e-6 * i, 0.0, 0.0)))
+    return out
+
+
+def make_batch(fs, doms):
+    b = fs.Batch([(d["method"], *d["dims"], d["tau"], d["force"]) for d in doms])
+    for i, d in enumerate(doms):
+        b.set_flags(i, d["flags"])
+        if d["inlet"] is not None:
+            b.set_inlet_profile(i, d["inlet"])
+        b.reinit_from_macro(i, *d["fields"])
+    return b
+
+
+def single(fs, d, steps):
+    with fs.Solver(d["method"], *d["dims"], d["tau"], d["force"]) as s:
+        s.set_flags(d["flags"])
+        if d["inlet"] is not None:
+            s.set_inlet_profile(d["inlet"])
+        s.reinit_from_macro(*d["fields"])
+        s.step(steps)
+        return s.macros(), s.state()
+
+
+@pytest.mark.parametrize("steps", [1, 2, 35])
+@pytest.mark.parametrize("launch", [(0, 0, True), (512, 256, True), (256, 64, False)])
+def test_batch_matches_single_domains(fs, steps, launch):
+    doms = small_domains()
+    with make_batch(fs, doms) as b:
+        b.set_launch(*launch)
+        b.step(steps)
+        for i, d in enumerate(doms):
+            mac, st = single(fs, d, steps)
+            for k in range(4):
+                np.testing.assert_array_equal(b.macro(i, k), mac[k], err_msg=f"dom {i} macro {k}")
+            np.testing.assert_array_equal(b.state(i), st, err_msg=f"dom {i} state")
+
+
+def test_batch_against_oracle(fs, oracle):
+    doms = small_domains()[:3]
+    with make_batch(fs, doms) as b:
+        b.step(20)
+        for i, d in enumerate(doms):
+            o = oracle.Oracle(d["method"], *d["dims"], d["tau"], d["force"])
+            o.set_flags(d["flags"])
+            if d["inlet"] is not None:
+                o.set_inlet(d["inlet"])
+            o.reinit(*d["fields"])
+            o.step(20)
+            np.testing.assert_array_equal(b.state(i), o.state())
+
+
+def test_batch_split_steps_and_autotune(fs):
+    doms = small_domains()
+    with make_batch(fs, doms) as b:
+        before = [b.state(i) for i in range(len(doms))]
+        p, h = b.autotune(4)
+        assert p >= 0 and h >= 0
+        for i in range(len(doms)):  # autotune restores the state
+            np.testing.assert_array_equal(b.state(i), before[i])
+        b.step(17)
+        b.step(20)
+        for i, d in enumerate(doms):
+            mac, st = single(fs, d, 37)
+            np.testing.assert_array_equal(b.state(i), st)
+
+
+def test_batch_rejects_bad_input(fs):
+    from myfluidsimulation_amd.capi import FluidSimError
+
+    with pytest.raises(FluidSimError):
+        fs.Batch([(1, 1, 8, 8, 0.6)])
+    nx, ny, nz = 33, 10, 10  # outlet at x = 32 starts a 32-wide HOME tile
+    with fs.Batch([(1, nx, ny, nz, 0.6)]) as b:
+        with pytest.raises(FluidSimError):
+            b.set_flags(0, wl.channel_flags(nx, ny, nz))
+        b.step(1)
+        with pytest.raises(FluidSimError):
+            b.reinit_from_macro(0, np.ones(nx * ny * nz, np.float32))

@@ -1,0 +1,160 @@
+"""CPU: pins the differentiable FP64 torch formulation (tests/torch_lbm.py),
+which the adjoint GPU tests use as their gradient reference.
+
+* fields after T steps against the FP64 oracle, every method, with walls,
+  obstacle, inlet, outlet, forcing, Hybrid band rates and HOME's reference
+  bounce table;
+* the drag of every step against the bounced-back populations the FP64
+  oracle holds after that step (an independent reading of D^n);
+* HOME's drag and fields against BGK's at tau = 1, where both rebuild f*
+  as feq(rho, u) from the same moments and so follow one trajectory;
+* autograd against central differences (FP64) on a small case.
+"""
+import numpy as np
+import pytest
+import torch
+
+import pyoracle
+from myfluidsimulation_amd import workloads as wl
+from torch_lbm import BGK, HOME, HYBRID, OPP, TorchLBM, E
+
+NX, NY, NZ = 14, 10, 8
+
+
+def make_case(kind, method, seed=3):
+    rng = np.random.default_rng(seed)
+    n = NX * NY * NZ
+    if kind == "box":
+        rho, ux, uy, uz = wl.taylor_green_3d(NX, NY, NZ, 0.04)
+        solid = np.zeros((NZ, NY, NX), np.uint8)
+        solid[2:4, 3:6, 4:7] = 1
+        flags = solid.reshape(-1).copy()
+        inlet = np.zeros(NY * NZ, np.float32)
+    else:
+        obst = np.zeros((NZ, NY, NX), np.uint8)
+        obst[3:5, 4:6, 4:6] = 1
+        flags = wl.channel_flags(NX, NY, NZ, obst.reshape(-1))
+        rho, ux, uy, uz = wl.uniform_flow(n, 0.04)
+        ux = (ux + 0.003 * rng.standard_normal(n)).astype(np.float32)
+        uy = (0.003 * rng.standard_normal(n)).astype(np.float32)
+        inlet = (0.04 + 0.005 * rng.standard_normal(NY * NZ)).astype(np.float32)
+    if method == HYBRID:
+        band = pyoracle.band((flags & 1).astype(np.uint8), NX, NY, NZ, 1)
+        flags = flags | (band * 2).astype(np.uint8)
+    return flags, (rho, ux, uy, uz), inlet
+
+
+def torch_run(method, tau, force, flags, fields, inlet, T, omegas=None, bounce=0):
+    ref = TorchLBM(method, NX, NY, NZ, tau, force, flags, omegas=omegas, bounce=bounce)
+    sh = (NZ, NY, NX)
+    d = lambda a: torch.tensor(np.asarray(a, np.float64)).reshape(sh)  # noqa: E731
+    rho, ux, uy, uz = fields
+    return ref.run(d(rho), torch.stack([d(ux), d(uy), d(uz)]),
+                   torch.tensor(np.asarray(inlet, np.float64)).reshape(NZ, NY), T)
+
+
+def oracle(method, tau, force, flags, fields, inlet, omegas=None, bounce=0):
+    o = pyoracle.Oracle(method, NX, NY, NZ, tau, force, precision=pyoracle.F64)
+    o.set_flags(flags)
+    o.set_inlet(inlet)
+    if omegas:
+        o.set_omegas(*omegas)
+    if bounce:
+        o.set_bounce(bounce)
+    o.reinit(*fields)
+    return o
+
+
+@pytest.mark.parametrize("kind", ["box", "channel"])
+@pytest.mark.parametrize("method,omegas,bounce,force", [
+    (BGK, None, 0, (0.0, 0.0, 0.0)),
+    (BGK, None, 0, (2e-5, -1e-5, 0.0)),
+    (HYBRID, (1 / 0.7, 1.2, 1.1), 0, (0.0, 0.0, 0.0)),
+    (HOME, None, 0, (1e-5, 0.0, 0.0)),
+    (HOME, None, 1, (0.0, 0.0, 0.0)),
+])
+def test_fields_match_fp64_oracle(kind, method, omegas, bounce, force):
+    tau, T = 0.7, 9
+    flags, fields, inlet = make_case(kind, method)
+    rho, u, _ = torch_run(method, tau, force, flags, fields, inlet, T, omegas, bounce)
+    o = oracle(method, tau, force, flags, fields, inlet, omegas, bounce)
+    o.step(T)
+    mo = o.macros(np.float64)
+    np.testing.assert_allclose(rho.reshape(-1).numpy(), mo[0], rtol=0, atol=1e-13)
+    for k in range(3):
+        np.testing.assert_allclose(u[k].reshape(-1).numpy(), mo[k + 1], rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("method", [BGK, HYBRID])
+def test_drag_matches_bounced_populations(method):
+    tau, T = 0.7, 6
+    flags, fields, inlet = make_case("channel", method)
+    omegas = (1 / tau, 1.2, 1.1) if method == HYBRID else None
+    _, _, drag = torch_run(method, tau, (0.0, 0.0, 0.0), flags, fields, inlet, T, omegas)
+    o = oracle(method, tau, (0.0, 0.0, 0.0), flags, fields, inlet, omegas)
+    solid = (flags.reshape(NZ, NY, NX) & 1) != 0
+    for n in range(T):
+        o.step(1)
+        f = o.state(np.float64).reshape(19, NZ, NY, NX)
+        expect = 0.0
+        for q in range(1, 19):
+            if E[q, 0] == 0:
+                continue
+            nb_solid = np.roll(solid, (-E[q, 2], -E[q, 1], -E[q, 0]), axis=(0, 1, 2))
+            link = (~solid) & nb_solid
+            # f*_q(x) came back as f_{opp q}(x) at the next time level
+            expect += 2.0 * E[q, 0] * f[OPP[q]][link].sum()
+        assert drag[n].item() == pytest.approx(expect, rel=1e-12, abs=1e-14)
+    assert abs(drag[-1].item()) > 1e-4  # the case has a real drag
+
+
+def test_home_equals_bgk_at_tau_one():
+    # (no forcing: BGK shifts u by F already in step 0, HOME from step 1 on;
+    # outlet and inlet cells start as BGK treats them from step 0 on: equal
+    # to their feeders, and at (1, u_in, 0, 0))
+    T = 8
+    flags, fields, inlet = make_case("channel", BGK)
+    fields = [np.asarray(f).reshape(NZ, NY, NX).copy() for f in fields]
+    for f in fields:
+        f[:, :, NX - 1] = f[:, :, NX - 2]
+    at_in = (flags.reshape(NZ, NY, NX)[:, :, 0] & 4) != 0
+    for k, v in enumerate((1.0, inlet.reshape(NZ, NY), 0.0, 0.0)):
+        fields[k][:, :, 0] = np.where(at_in, v, fields[k][:, :, 0])
+    fields = [f.reshape(-1) for f in fields]
+    rb, ub, db = torch_run(BGK, 1.0, (0, 0, 0), flags, fields, inlet, T)
+    rh, uh, dh = torch_run(HOME, 1.0, (0, 0, 0), flags, fields, inlet, T)
+    assert torch.allclose(db, dh, rtol=0, atol=1e-13)
+    # BGK reports the moments entering its last step, HOME the end moments
+    rb2, ub2, _ = torch_run(BGK, 1.0, (0, 0, 0), flags, fields, inlet, T + 1)
+    assert torch.allclose(rb2, rh, rtol=0, atol=1e-13)
+    assert torch.allclose(ub2, uh, rtol=0, atol=1e-13)
+    assert not torch.allclose(rb, rh, rtol=0, atol=1e-9)
+
+
+@pytest.mark.parametrize("method", [BGK, HOME])
+def test_autograd_matches_central_differences(method):
+    tau, T = 0.7, 4
+    flags, fields, inlet = make_case("channel", method)
+    ref = TorchLBM(method, NX, NY, NZ, tau, (0, 0, 0), flags)
+    sh = (NZ, NY, NX)
+    rng = np.random.default_rng(9)
+    u0 = torch.tensor(np.stack([np.asarray(f, np.float64).reshape(sh) for f in fields[1:]]))
+    rho0 = torch.tensor(np.asarray(fields[0], np.float64).reshape(sh))
+    uin = torch.tensor(np.asarray(inlet, np.float64).reshape(NZ, NY))
+    tgt = torch.tensor(rng.standard_normal((3,) + sh) * 0.01)
+    wd = torch.tensor(rng.standard_normal(T))
+
+    def J(u, ui):
+        _, uT, d = ref.run(rho0, u, ui, T)
+        return 0.5 * ((uT - tgt) ** 2).sum() + (d * wd).sum()
+
+    u = u0.clone().requires_grad_()
+    ui = uin.clone().requires_grad_()
+    J(u, ui).backward()
+    for _ in range(3):
+        v = torch.tensor(rng.standard_normal(u0.shape))
+        w = torch.tensor(rng.standard_normal(uin.shape))
+        eps = 1e-6
+        fd = (J(u0 + eps * v, uin + eps * w) - J(u0 - eps * v, uin - eps * w)).item() / (2 * eps)
+        ad = (u.grad * v).sum().item() + (ui.grad * w).sum().item()
+        assert ad == pytest.approx(fd, rel=1e-7)
