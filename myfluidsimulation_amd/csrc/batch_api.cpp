@@ -3,6 +3,10 @@
 // A batch owns the device buffers of all its domains in one allocation, the
 // device descriptor tables and tile work lists read by batch.hip, and a
 // captured hipGraph of STEPS_PER_GRAPH steps that replays the step loop.
+// The population domains and the HOME domains never read each other, so a
+// run of steps is two independent launch chains: populations on the batch's
+// stream, HOME on a second stream forked from it and joined back at the end,
+// and the two kinds of domain share the chip instead of taking turns.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -50,6 +54,8 @@ struct fs_batch {
   uint16_t* d_popTile = nullptr;
   uint16_t* d_homeTile = nullptr;
   hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;  // HOME chain
+  hipEvent_t fork = nullptr, join = nullptr;
   int parity = 0;  // shared by every domain: they all step together
   int popBlocks = 0, homeBlocks = 0;  // 0 = one workgroup per tile
   bool useGraph = true;
@@ -76,6 +82,10 @@ void free_batch(fs_batch* b) {
   (void)hipFree(b->d_home);
   (void)hipFree(b->d_popTile);
   (void)hipFree(b->d_homeTile);
+  if (b->side) (void)hipStreamSynchronize(b->side);
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  if (b->join) (void)hipEventDestroy(b->join);
+  if (b->side) (void)hipStreamDestroy(b->side);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }
@@ -105,21 +115,30 @@ int upload_flags(fs_batch* b, Dom& D) {
   return FS_OK;
 }
 
-int launch_step(fs_batch* b, bool macro) {
-  const bool odd = b->parity != 0;
-  FS_HIP(fs::launch_batch_pop(b->d_pop, b->d_popTile, b->popTiles, odd, macro, b->popBlocks,
-                              b->stream));
-  FS_HIP(fs::launch_batch_home(b->d_home, b->d_homeTile, b->homeTiles, b->parity, b->homeBlocks,
-                               b->stream));
-  b->parity ^= 1;
+// n steps: the population chain on the batch's stream (macros written by
+// the last step when `macro`), the HOME chain on the side stream
+int launch_steps(fs_batch* b, int n, bool macro) {
+  if (n <= 0) return FS_OK;
+  FS_HIP(hipEventRecord(b->fork, b->stream));
+  FS_HIP(hipStreamWaitEvent(b->side, b->fork, 0));
+  for (int i = 0; i < n; ++i) {
+    const bool odd = ((b->parity + i) & 1) != 0;
+    FS_HIP(fs::launch_batch_pop(b->d_pop, b->d_popTile, b->popTiles, odd, macro && i == n - 1,
+                                b->popBlocks, b->stream));
+  }
+  for (int i = 0; i < n; ++i)
+    FS_HIP(fs::launch_batch_home(b->d_home, b->d_homeTile, b->homeTiles, (b->parity + i) & 1,
+                                 b->homeBlocks, b->side));
+  FS_HIP(hipEventRecord(b->join, b->side));
+  FS_HIP(hipStreamWaitEvent(b->stream, b->join, 0));
+  b->parity = (b->parity + n) & 1;
   return FS_OK;
 }
 
 int build_graph(fs_batch* b) {
   drop_graph(b);
   FS_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  int rc = FS_OK;
-  for (int i = 0; i < kStepsPerGraph && rc == FS_OK; ++i) rc = launch_step(b, false);
+  const int rc = launch_steps(b, kStepsPerGraph, false);
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(b->stream, &g);
   b->parity = 0;
@@ -146,9 +165,7 @@ int enqueue(fs_batch* b, int nsteps) {
       left -= kStepsPerGraph;
     }
   }
-  for (int i = 0; i < left; ++i)
-    if (int rc = launch_step(b, i == left - 1)) return rc;
-  return FS_OK;
+  return launch_steps(b, left, true);
 }
 
 }  // namespace
@@ -202,7 +219,10 @@ int fs_batch_create(const fs_domain_desc* descs, int ndomains, fs_batch** out) {
       offs.push_back(carve(P));
       offs.push_back(carve(sizeof(float) * size_t(s.ny) * s.nz));
     }
-    if (hipStreamCreate(&b->stream) != hipSuccess) return bail(fail(FS_ERR_DEVICE, "stream"));
+    if (hipStreamCreate(&b->stream) != hipSuccess || hipStreamCreate(&b->side) != hipSuccess ||
+        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess)
+      return bail(fail(FS_ERR_DEVICE, "stream"));
     if (hipMalloc((void**)&b->arena, bytes) != hipSuccess)
       return bail(fail(FS_ERR_ALLOC, "batch arena of %zu bytes", bytes));
     std::vector<uint16_t> popTile, homeTile;

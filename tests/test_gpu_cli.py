@@ -1,0 +1,139 @@
+"""GPU: the drop-in executables end to end.  fluidsim_legacy / _home /
+_hybrid run a small obstacle case (STL voxelized on the GPU, slab density
+perturbation, forcing, substeps, VTK every few steps); the same run is
+replayed on the oracle (its STL loader, voxelizer, band, density profile,
+solver, speed and VTK writer) and every VTK file must match byte for byte,
+and the [vtk] / [run] lines must be the reference's (sim_runner.hpp:154-168,
+252-264).  The channel extensions (inlet, outlet, y/z walls) are covered the
+same way."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from myfluidsimulation_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+NX, NY, NZ = 24, 16, 12
+TAU, FX = 0.6, 1e-5
+STEPS, SUB, EVERY = 6, 2, 3
+SLAB = dict(axis=0, amp=2e-3, center=0.4, width=0.3)
+
+
+def f32s(v):
+    """text that parses back (stof) to the same float32"""
+    return repr(float(np.float32(v)))
+
+
+def run_cli(name, args, tmp):
+    exe = os.path.join(ROOT, "myfluidsimulation_amd", "bin", name)
+    r = subprocess.run([exe] + args, capture_output=True, text=True, timeout=120, cwd=tmp)
+    assert r.returncode == 0, r.stderr
+    return r.stdout
+
+
+def case_args(stl, scale, trans, vtk_dir, extra=()):
+    return (["--nx", str(NX), "--ny", str(NY), "--nz", str(NZ), "--tau", f32s(TAU),
+             "--force-x", f32s(FX), "--stl", stl, "--voxel-scale", f32s(scale),
+             "--voxel-translate"] + [f32s(t) for t in trans] +
+            ["--init", "slab", "--axis", "x", "--amp", f32s(SLAB["amp"]),
+             "--slab-center", f32s(SLAB["center"]), "--width", f32s(SLAB["width"]),
+             "--steps", str(STEPS), "--substeps", str(SUB), "--vtk-every", str(EVERY),
+             "--vtk-dir", vtk_dir] + list(extra))
+
+
+def replay(oracle, method, stl, scale, trans, out_dir, tag, channel=False, d0=None, inlet_u=0.04):
+    tris, bmin, bmax = oracle.load_stl(stl)
+    solid = oracle.voxelize(tris, bmin, bmax, NX, NY, NZ, scale, trans)
+    if channel:
+        flags = wl.channel_flags(NX, NY, NZ, solid)
+    else:
+        flags = solid.copy()
+    if d0 is not None:
+        flags = flags | (oracle.band((flags & 1).astype(np.uint8), NX, NY, NZ, d0) * 2).astype(
+            np.uint8)
+    o = oracle.Oracle(method, NX, NY, NZ, np.float32(TAU), (np.float32(FX), 0.0, 0.0))
+    o.set_flags(flags)
+    n = NX * NY * NZ
+    rho = oracle.rho_slab(NX, NY, NZ, SLAB["axis"], SLAB["amp"], SLAB["center"], SLAB["width"])
+    if channel:
+        o.set_inlet(np.full(NY * NZ, inlet_u, np.float32))
+        o.reinit(rho, np.full(n, inlet_u, np.float32), np.zeros(n, np.float32),
+                 np.zeros(n, np.float32))
+    else:
+        o.reinit(rho)
+    os.makedirs(out_dir, exist_ok=True)
+
+    def dump(step):
+        r, u, v, w = o.macros()
+        oracle.write_scalar_vtk(os.path.join(out_dir, f"rho_{tag}_{step:06d}.vtk"), r, NX, NY, NZ)
+        oracle.write_scalar_vtk(os.path.join(out_dir, f"speed_{tag}_{step:06d}.vtk"),
+                                oracle.speed(u, v, w), NX, NY, NZ)
+        oracle.write_vector_vtk(os.path.join(out_dir, f"vel_{tag}_{step:06d}.vtk"), u, v, w,
+                                NX, NY, NZ)
+
+    dump(0)
+    for step in range(1, STEPS + 1):
+        o.step(SUB)
+        if step % EVERY == 0:
+            dump(step)
+
+
+def compare_dirs(got, want):
+    names = sorted(os.listdir(want))
+    assert sorted(os.listdir(got)) == names
+    assert len(names) == 3 * (STEPS // EVERY + 1)
+    for nm in names:
+        with open(os.path.join(got, nm), "rb") as a, open(os.path.join(want, nm), "rb") as b:
+            assert a.read() == b.read(), nm
+
+
+@pytest.fixture
+def obstacle(tmp_path):
+    stl = str(tmp_path / "body.stl")
+    scale, trans = wl.obstacle_stl(stl, NX, NY, NZ, (8.0, 8.0, 6.0), 3.5, seed=4242)
+    return stl, scale, trans
+
+
+def run_line(tag, extra=""):
+    return (f"[run] {tag}  {extra}Nx={NX} Ny={NY} Nz={NZ}  tau={np.float32(TAU):.4g}  "
+            f"force=({np.float32(FX):.3g},{0.0:.3g},{0.0:.3g})  steps={STEPS} substeps={SUB}")
+
+
+@pytest.mark.parametrize("name,method,tag", [("fluidsim_legacy", 0, "legacy"),
+                                             ("fluidsim_home", 1, "home")])
+def test_single_solver_cli(oracle, obstacle, tmp_path, name, method, tag):
+    stl, scale, trans = obstacle
+    got = str(tmp_path / "gpu")
+    out = run_cli(name, case_args(stl, scale, trans, got), str(tmp_path))
+    assert out.splitlines() == [f"[vtk] {tag}: dir={got} every={EVERY}", run_line(tag)]
+    want = str(tmp_path / "oracle")
+    replay(oracle, method, stl, scale, trans, want, tag)
+    compare_dirs(got, want)
+
+
+def test_hybrid_cli(oracle, obstacle, tmp_path):
+    stl, scale, trans = obstacle
+    got = str(tmp_path / "gpu")
+    out = run_cli("fluidsim_hybrid", case_args(stl, scale, trans, got, ["--hybrid-band", "2"]),
+                  str(tmp_path))
+    assert out.splitlines() == [f"[vtk] hybrid: dir={got} every={EVERY}",
+                                run_line("hybrid(B0)", "d0=2  ")]
+    want = str(tmp_path / "oracle")
+    replay(oracle, 2, stl, scale, trans, want, "hybrid", d0=2)
+    compare_dirs(got, want)
+
+
+@pytest.mark.parametrize("name,method,tag", [("fluidsim_legacy", 0, "legacy"),
+                                             ("fluidsim_home", 1, "home")])
+def test_channel_extensions_cli(oracle, obstacle, tmp_path, name, method, tag):
+    stl, scale, trans = obstacle
+    got = str(tmp_path / "gpu")
+    extra = ["--inlet-u", f32s(0.04), "--outlet", "zero-gradient", "--walls", "yz"]
+    run_cli(name, case_args(stl, scale, trans, got, extra), str(tmp_path))
+    want = str(tmp_path / "oracle")
+    replay(oracle, method, stl, scale, trans, want, tag, channel=True)
+    compare_dirs(got, want)
