@@ -154,15 +154,33 @@ __device__ __forceinline__ void home_tile(const Geo& g, const Phys& ph, const Ho
 #pragma unroll
   for (int k = 0; k < 10; ++k) mPrev.v[k] = 0.0f;
 
-  for (int p = z0 - 1; p <= z0 + ZCe; ++p) {
+  // The moments of source plane p+1 are loaded while plane p is computed
+  // (one plane of register prefetch), so HBM latency overlaps the
+  // reconstruction and the two barriers of each plane.
+  Mom nxtC, nxtH;
+  uint8_t nxtFl = 0;
+  auto load_plane = [&](int p) {
     const int pw = (p + g.nz) % g.nz;
-    Mom cur;
-    uint8_t flCur = 0;
     if (active) {
       const uint32_t c = (uint32_t(pw) * g.ny + y) * g.nx + x;
 #pragma unroll
-      for (int k = 0; k < 10; ++k) cur.v[k] = a.m[size_t(k) * P + c];
-      flCur = a.flags[c];
+      for (int k = 0; k < 10; ++k) nxtC.v[k] = a.m[size_t(k) * P + c];
+      nxtFl = a.flags[c];
+    }
+    if (side >= 0) {
+      const uint32_t c = (uint32_t(pw) * g.ny + gy) * g.nx + gx;
+#pragma unroll
+      for (int k = 0; k < 10; ++k) nxtH.v[k] = a.m[size_t(k) * P + c];
+    }
+  };
+  load_plane(z0 - 1);
+
+  for (int p = z0 - 1; p <= z0 + ZCe; ++p) {
+    const Mom cur = nxtC;
+    const uint8_t flCur = nxtFl;
+    const Mom hm = nxtH;
+    if (p < z0 + ZCe) load_plane(p + 1);
+    if (active) {
       const HomeSrc h(cur, omm);
       for_q<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
@@ -170,10 +188,6 @@ __device__ __forceinline__ void home_tile(const Geo& g, const Phys& ph, const Ho
       });
     }
     if (side >= 0) {
-      const uint32_t c = (uint32_t(pw) * g.ny + gy) * g.nx + gx;
-      Mom hm;
-#pragma unroll
-      for (int k = 0; k < 10; ++k) hm.v[k] = a.m[size_t(k) * P + c];
       const HomeSrc h(hm, omm);
       const int row = hy + 1, col = hx + 1;
       if (side == 0)
