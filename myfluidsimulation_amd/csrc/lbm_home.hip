@@ -53,16 +53,17 @@ template <int TX, int TY, int ZC, int BB>
 __global__ void __launch_bounds__(TX * TY) k_home_tiled(Geo g, Phys ph, HomeLaunch a) {
   __shared__ float fs[19 * TileLds<TX, TY>::PL];
   const int tilesX = (g.nx + TX - 1) / TX;
-  const int chunks = (g.nz + ZC - 1) / ZC;
   // Workgroups the dispatcher sends to one XCD (every 8th) get one
-  // contiguous run of tiles, z-chunk fastest, so the planes and halo rows
-  // that neighbouring tiles both read stay in that XCD's L2.
+  // contiguous run of tiles, x fastest, then y, then the z-chunk: the tiles
+  // an XCD runs at the same time are neighbours in one z-slab, so the halo
+  // rows they share are read once into that XCD's L2.
   uint32_t L0 = blockIdx.x;
   const uint32_t total = gridDim.x;
   if ((total & 7u) == 0) L0 = (L0 & 7u) * (total >> 3) + (L0 >> 3);
-  const int bz = int(L0 % uint32_t(chunks));
-  const int rest = int(L0 / uint32_t(chunks));
-  home_tile<TX, TY, ZC, BB>(g, ph, a, rest % tilesX, rest / tilesX, bz, fs);
+  const int tilesY = (g.ny + TY - 1) / TY;
+  const int bx = int(L0 % uint32_t(tilesX));
+  const int rest = int(L0 / uint32_t(tilesX));
+  home_tile<TX, TY, ZC, BB>(g, ph, a, bx, rest % tilesY, rest / tilesY, fs);
 }
 
 // kern_reset_mom / kern_reinit_from_macro (lbm3d_home.cu:83-137): both buffers
