@@ -236,6 +236,8 @@ def main():
     ap.add_argument("--workload", default="channel",
                     choices=["channel", "tgv512", "channel768", "batch64"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the 64-domain batched sweep summary of the channel workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -308,6 +310,13 @@ def main():
                      "kernel": "k_pop_step (BGK, AA pattern)",
                      "measured_copy_ceiling_GBps": round(ceiling, 1)},
     }
+    if args.workload == "channel" and not args.no_batch:
+        b = batch_bench(300, 20)
+        line["batched"] = {k: b[k] for k in ("value", "unit", "ms_per_step",
+                                             "sequential_sync_step1_MLUPS",
+                                             "sequential_async_MLUPS", "speedup_vs_sync_step1",
+                                             "speedup_vs_async", "finite")}
+        line["batched"]["workload"] = b["config"]["workload"]
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(case)
     print(json.dumps(line))
