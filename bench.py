@@ -9,9 +9,12 @@ headline `value`), HOME and Hybrid(B0) band 2 each run W untimed then K
 timed steps; one "step" is one lattice time step over the whole domain.
 
 Other workloads (--workload), the large single-domain roofline runs:
+  tgv256      256^3 periodic Taylor-Green vortex, BGK and HOME
   tgv512      512^3 periodic Taylor-Green vortex, BGK and HOME
   channel768  768x256x256 channel, icosphere radius 40, Hybrid bands 1, 2, 4
               (and BGK, HOME)
+  batch64     the 64-domain batched sweep against the same runs one by one
+The default channel line also carries a shorter run of the batched sweep.
 
 Timing: HIP events recorded on the solver's stream around the K steps
 (fs_solver_timed_steps); every step is one kernel launch, so the event time
@@ -234,7 +237,7 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", default="channel",
-                    choices=["channel", "tgv512", "channel768", "batch64"])
+                    choices=["channel", "tgv256", "tgv512", "channel768", "batch64"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the 64-domain batched sweep summary of the channel workload")
@@ -261,6 +264,11 @@ def main():
         steps, warmup = args.steps or 5000, 100 if args.warmup is None else args.warmup
         desc = ("obstacle channel 384x128x128, tau 0.56, inlet u 0.05, zero-gradient outlet, "
                 f"y/z walls, icosphere R20 seed {SEED}; value = BGK, also HOME and Hybrid band 2")
+    elif args.workload == "tgv256":
+        case = tgv_case(256)
+        runs = [("bgk", 0, 0), ("home", 1, 0)]
+        steps, warmup = args.steps or 400, 50 if args.warmup is None else args.warmup
+        desc = "Taylor-Green vortex 256^3 periodic, tau 0.6, u0 0.05; value = BGK, also HOME"
     elif args.workload == "tgv512":
         case = tgv_case(512)
         runs = [("bgk", 0, 0), ("home", 1, 0)]
