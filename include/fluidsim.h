@@ -277,6 +277,10 @@ int fs_band(const uint8_t* host_solid, int nx, int ny, int nz, int d0,
 /* launch_speed (field_ops.cu:25-28) */
 int fs_speed(const float* d_u, const float* d_v, const float* d_w,
              float* d_out, size_t n);
+/* launch_diff / launch_shift (field_ops.cu:20-23,29-32,34-41): out = a - b,
+ * out = a + bias (device, n floats) */
+int fs_diff(const float* d_a, const float* d_b, float* d_out, size_t n);
+int fs_shift(const float* d_a, float bias, float* d_out, size_t n);
 /* make_rho_gauss / make_rho_slab (init_profiles.cu:31-45) */
 int fs_rho_gauss(float* d_rho, int nx, int ny, int nz, float cx, float cy,
                  float cz, float sigma, float amp);
@@ -293,6 +297,14 @@ int fs_write_scalar_vtk(const char* filename, const float* d_field, int nx,
 int fs_write_vector_vtk(const char* filename, const float* d_fx,
                         const float* d_fy, const float* d_fz, int nx, int ny,
                         int nz);
+/* Extension: the same files in the legacy format's BINARY encoding (same
+ * header with BINARY, values as big-endian float32): exact values, about a
+ * third of the size, no text formatting. */
+int fs_write_scalar_vtk_binary(const char* filename, const float* d_field,
+                               int nx, int ny, int nz);
+int fs_write_vector_vtk_binary(const char* filename, const float* d_fx,
+                               const float* d_fy, const float* d_fz, int nx,
+                               int ny, int nz);
 
 /* Device memory helpers for callers built without HIP headers. */
 int fs_device_malloc(void** d_ptr, size_t bytes);
@@ -326,6 +338,8 @@ void reinit_equilibrium_from_macro(const float* d_rho, const float* d_ux,
                                    float* d_f, int N);
 void launch_speed(const float* u, const float* v, const float* w, float* out,
                   int N);
+void launch_diff(const float* a, const float* b, float* out, int N);
+void launch_shift(const float* a, float bias, float* out, int N);
 
 #ifdef __cplusplus
 }

@@ -95,11 +95,15 @@ SIGNATURES = {
     "fs_voxelize": (_i, [_p, _i, _p, _p, _i, _i, _i, _f, _f, _f, _f, _i, _f, _p]),
     "fs_band": (_i, [_p, _i, _i, _i, _i, _p]),
     "fs_speed": (_i, [_p, _p, _p, _p, _sz]),
+    "fs_diff": (_i, [_p, _p, _p, _sz]),
+    "fs_shift": (_i, [_p, _f, _p, _sz]),
     "fs_rho_gauss": (_i, [_p, _i, _i, _i, _f, _f, _f, _f, _f]),
     "fs_rho_slab": (_i, [_p, _i, _i, _i, _i, _f, _f, _f]),
     "fs_reinit_equilibrium": (_i, [_p, _p, _p, _p, _p, _sz]),
     "fs_write_scalar_vtk": (_i, [C.c_char_p, _p, _i, _i, _i]),
     "fs_write_vector_vtk": (_i, [C.c_char_p, _p, _p, _p, _i, _i, _i]),
+    "fs_write_scalar_vtk_binary": (_i, [C.c_char_p, _p, _i, _i, _i]),
+    "fs_write_vector_vtk_binary": (_i, [C.c_char_p, _p, _p, _p, _i, _i, _i]),
     "fs_device_malloc": (_i, [C.POINTER(_p), _sz]),
     "fs_device_free": (_i, [_p]),
     "fs_memcpy_htod": (_i, [_p, _p, _sz]),
@@ -113,6 +117,8 @@ SIGNATURES = {
     "make_rho_slab": (None, [_p, _i, _i, _i, _i, _f, _f, _f]),
     "reinit_equilibrium_from_macro": (None, [_p, _p, _p, _p, _p, _i]),
     "launch_speed": (None, [_p, _p, _p, _p, _i]),
+    "launch_diff": (None, [_p, _p, _p, _i]),
+    "launch_shift": (None, [_p, _f, _p, _i]),
 }
 
 

@@ -26,6 +26,9 @@ bool read_stl(const char* path, std::vector<float>& xyz);
 bool vtk_scalar(const char* path, const float* h, int nx, int ny, int nz);
 bool vtk_vector(const char* path, const float* a, const float* b, const float* c, int nx, int ny,
                 int nz);
+bool vtk_scalar_binary(const char* path, const float* h, int nx, int ny, int nz);
+bool vtk_vector_binary(const char* path, const float* a, const float* b, const float* c, int nx,
+                       int ny, int nz);
 }  // namespace fs
 
 namespace {
@@ -669,6 +672,18 @@ int fs_speed(const float* u, const float* v, const float* w, float* out, size_t 
   return FS_OK;
 }
 
+int fs_diff(const float* a, const float* b, float* out, size_t n) {
+  if (n && (!a || !b || !out)) return fail(FS_ERR_INVALID, "null field");
+  FS_HIP(fs::launch_diff_k(a, b, out, n, nullptr));
+  return FS_OK;
+}
+
+int fs_shift(const float* a, float bias, float* out, size_t n) {
+  if (n && (!a || !out)) return fail(FS_ERR_INVALID, "null field");
+  FS_HIP(fs::launch_shift_k(a, bias, out, n, nullptr));
+  return FS_OK;
+}
+
 int fs_rho_gauss(float* rho, int nx, int ny, int nz, float cx, float cy, float cz, float sigma,
                  float amp) {
   if (!rho || nx < 1 || ny < 1 || nz < 1) return fail(FS_ERR_INVALID, "bad gauss arguments");
@@ -690,21 +705,24 @@ int fs_reinit_equilibrium(const float* rho, const float* ux, const float* uy, co
   return FS_OK;
 }
 
-int fs_write_scalar_vtk(const char* path, const float* d_field, int nx, int ny, int nz) {
+namespace {
+
+int write_scalar(const char* path, const float* d_field, int nx, int ny, int nz, bool binary) {
   if (!path || !d_field || nx < 1 || ny < 1 || nz < 1)
     return fail(FS_ERR_INVALID, "bad VTK arguments");
   return guarded([&]() -> int {
     const size_t n = size_t(nx) * ny * nz;
     std::vector<float> h(n);
     FS_HIP(hipMemcpy(h.data(), d_field, n * sizeof(float), hipMemcpyDeviceToHost));
-    if (!fs::vtk_scalar(path, h.data(), nx, ny, nz))
-      return fail(FS_ERR_IO, "failed to open VTK file %s", path);
+    const bool ok = binary ? fs::vtk_scalar_binary(path, h.data(), nx, ny, nz)
+                           : fs::vtk_scalar(path, h.data(), nx, ny, nz);
+    if (!ok) return fail(FS_ERR_IO, "failed to write VTK file %s", path);
     return FS_OK;
   });
 }
 
-int fs_write_vector_vtk(const char* path, const float* a, const float* b, const float* c, int nx,
-                        int ny, int nz) {
+int write_vector(const char* path, const float* a, const float* b, const float* c, int nx, int ny,
+                 int nz, bool binary) {
   if (!path || !a || !b || !c || nx < 1 || ny < 1 || nz < 1)
     return fail(FS_ERR_INVALID, "bad VTK arguments");
   return guarded([&]() -> int {
@@ -713,10 +731,32 @@ int fs_write_vector_vtk(const char* path, const float* a, const float* b, const 
     FS_HIP(hipMemcpy(h.data(), a, n * sizeof(float), hipMemcpyDeviceToHost));
     FS_HIP(hipMemcpy(h.data() + n, b, n * sizeof(float), hipMemcpyDeviceToHost));
     FS_HIP(hipMemcpy(h.data() + 2 * n, c, n * sizeof(float), hipMemcpyDeviceToHost));
-    if (!fs::vtk_vector(path, h.data(), h.data() + n, h.data() + 2 * n, nx, ny, nz))
-      return fail(FS_ERR_IO, "failed to open VTK file %s", path);
+    const float *x = h.data(), *y = h.data() + n, *z = h.data() + 2 * n;
+    const bool ok = binary ? fs::vtk_vector_binary(path, x, y, z, nx, ny, nz)
+                           : fs::vtk_vector(path, x, y, z, nx, ny, nz);
+    if (!ok) return fail(FS_ERR_IO, "failed to write VTK file %s", path);
     return FS_OK;
   });
+}
+
+}  // namespace
+
+int fs_write_scalar_vtk(const char* path, const float* d_field, int nx, int ny, int nz) {
+  return write_scalar(path, d_field, nx, ny, nz, false);
+}
+
+int fs_write_vector_vtk(const char* path, const float* a, const float* b, const float* c, int nx,
+                        int ny, int nz) {
+  return write_vector(path, a, b, c, nx, ny, nz, false);
+}
+
+int fs_write_scalar_vtk_binary(const char* path, const float* d_field, int nx, int ny, int nz) {
+  return write_scalar(path, d_field, nx, ny, nz, true);
+}
+
+int fs_write_vector_vtk_binary(const char* path, const float* a, const float* b, const float* c,
+                               int nx, int ny, int nz) {
+  return write_vector(path, a, b, c, nx, ny, nz, true);
 }
 
 int fs_device_malloc(void** p, size_t bytes) {
@@ -796,6 +836,12 @@ void reinit_equilibrium_from_macro(const float* d_rho, const float* d_ux, const 
 }
 void launch_speed(const float* u, const float* v, const float* w, float* out, int N) {
   die_on(fs_speed(u, v, w, out, N < 0 ? 0 : size_t(N)), "launch_speed");
+}
+void launch_diff(const float* a, const float* b, float* out, int N) {
+  die_on(fs_diff(a, b, out, N < 0 ? 0 : size_t(N)), "launch_diff");
+}
+void launch_shift(const float* a, float bias, float* out, int N) {
+  die_on(fs_shift(a, bias, out, N < 0 ? 0 : size_t(N)), "launch_shift");
 }
 
 }  // extern "C"

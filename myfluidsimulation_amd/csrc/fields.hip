@@ -21,6 +21,20 @@ __global__ void __launch_bounds__(kBlock) k_speed(const float* __restrict__ u,
   out[i] = sqrtf(a * a + b * b + c * c);
 }
 
+// kern_diff / kern_shift (field_ops.cu:20-23,34-37): a - b, a + bias
+__global__ void __launch_bounds__(kBlock) k_diff(const float* __restrict__ a,
+                                                 const float* __restrict__ b,
+                                                 float* __restrict__ out, size_t n) {
+  const size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < n) out[i] = a[i] - b[i];
+}
+
+__global__ void __launch_bounds__(kBlock) k_shift(const float* __restrict__ a, float bias,
+                                                  float* __restrict__ out, size_t n) {
+  const size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < n) out[i] = a[i] + bias;
+}
+
 __global__ void __launch_bounds__(kBlock) k_rho_gauss(float* rho, int nx, int ny, int nz, float cx,
                                                       float cy, float cz, float sigma, float amp) {
   const size_t i = size_t(blockIdx.x) * kBlock + threadIdx.x;
@@ -67,6 +81,18 @@ hipError_t launch_speed_k(const float* u, const float* v, const float* w, float*
                           hipStream_t st) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_speed, grid1d(n), dim3(kBlock), 0, st, u, v, w, out, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_diff_k(const float* a, const float* b, float* out, size_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_diff, grid1d(n), dim3(kBlock), 0, st, a, b, out, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_shift_k(const float* a, float bias, float* out, size_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_shift, grid1d(n), dim3(kBlock), 0, st, a, bias, out, n);
   return hipGetLastError();
 }
 

@@ -139,6 +139,8 @@ hipError_t launch_home_fill(const Geo& g, float* m, float* mn, const float* rho,
 // fields
 hipError_t launch_speed_k(const float* u, const float* v, const float* w, float* out, size_t n,
                           hipStream_t st);
+hipError_t launch_diff_k(const float* a, const float* b, float* out, size_t n, hipStream_t st);
+hipError_t launch_shift_k(const float* a, float bias, float* out, size_t n, hipStream_t st);
 hipError_t launch_rho_gauss(float* rho, int nx, int ny, int nz, float cx, float cy, float cz,
                             float sigma, float amp, hipStream_t st);
 hipError_t launch_rho_slab(float* rho, int nx, int ny, int nz, int axis, float amp, float center,

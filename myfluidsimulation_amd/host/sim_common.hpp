@@ -10,6 +10,8 @@
 //   --inlet-u U              equilibrium inlet at x=0, uniform initial flow U
 //   --outlet zero-gradient   zero-gradient outlet at x=nx-1
 //   --walls yz               solid wall planes at y=0,ny-1 and z=0,nz-1
+//   --vtk-format binary      VTK files in the legacy BINARY encoding
+//                            (big-endian float32) instead of ASCII
 #pragma once
 
 #include <algorithm>
@@ -62,6 +64,7 @@ struct RunConfig {
   float inletU = 0.0f;
   bool outlet = false;
   bool wallsYZ = false;
+  bool vtkBinary = false;
 };
 
 struct HybridConfig {
@@ -132,6 +135,8 @@ inline void parse_common_args(int argc, char** argv, DomainConfig& dom, Obstacle
       run.outlet = std::string(argv[++i]) == "zero-gradient";
     } else if (a == "--walls" && left >= 1) {
       run.wallsYZ = std::string(argv[++i]) == "yz";
+    } else if (a == "--vtk-format" && left >= 1) {
+      run.vtkBinary = std::string(argv[++i]) == "binary";
     }
   }
   dom.Nx = std::max(dom.Nx, 8);

@@ -18,10 +18,11 @@ struct VtkDumper {
   const DomainConfig& dom;
   std::optional<std::filesystem::path> dir;
   std::string tag;
+  bool binary;
   float* d_speed = nullptr;
 
   VtkDumper(Solver& s, const DomainConfig& d, const RunConfig& run, std::string t)
-      : solver(s), dom(d), dir(prepare_vtk_dir(run)), tag(std::move(t)) {
+      : solver(s), dom(d), dir(prepare_vtk_dir(run)), tag(std::move(t)), binary(run.vtkBinary) {
     if (dir)
       fs_throw_if(fs_device_malloc((void**)&d_speed,
                                    sizeof(float) * size_t(dom.Nx) * dom.Ny * dom.Nz),
@@ -34,12 +35,21 @@ struct VtkDumper {
     if (!dir) return;
     const int N = dom.Nx * dom.Ny * dom.Nz;
     launch_speed(solver.d_u(), solver.d_v(), solver.d_w(), d_speed, N);
-    write_scalar_vtk(vtk_path(*dir, "rho_" + tag, step).c_str(), solver.d_rho(), dom.Nx, dom.Ny,
-                     dom.Nz);
-    write_scalar_vtk(vtk_path(*dir, "speed_" + tag, step).c_str(), d_speed, dom.Nx, dom.Ny,
-                     dom.Nz);
-    write_vector_vtk(vtk_path(*dir, "vel_" + tag, step).c_str(), solver.d_u(), solver.d_v(),
-                     solver.d_w(), dom.Nx, dom.Ny, dom.Nz);
+    const std::string r = vtk_path(*dir, "rho_" + tag, step), s = vtk_path(*dir, "speed_" + tag, step),
+                      v = vtk_path(*dir, "vel_" + tag, step);
+    if (binary) {
+      fs_throw_if(fs_write_scalar_vtk_binary(r.c_str(), solver.d_rho(), dom.Nx, dom.Ny, dom.Nz),
+                  "write_scalar_vtk_binary");
+      fs_throw_if(fs_write_scalar_vtk_binary(s.c_str(), d_speed, dom.Nx, dom.Ny, dom.Nz),
+                  "write_scalar_vtk_binary");
+      fs_throw_if(fs_write_vector_vtk_binary(v.c_str(), solver.d_u(), solver.d_v(), solver.d_w(),
+                                             dom.Nx, dom.Ny, dom.Nz),
+                  "write_vector_vtk_binary");
+      return;
+    }
+    write_scalar_vtk(r.c_str(), solver.d_rho(), dom.Nx, dom.Ny, dom.Nz);
+    write_scalar_vtk(s.c_str(), d_speed, dom.Nx, dom.Ny, dom.Nz);
+    write_vector_vtk(v.c_str(), solver.d_u(), solver.d_v(), solver.d_w(), dom.Nx, dom.Ny, dom.Nz);
   }
 };
 

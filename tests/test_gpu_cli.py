@@ -137,3 +137,23 @@ def test_channel_extensions_cli(oracle, obstacle, tmp_path, name, method, tag):
     want = str(tmp_path / "oracle")
     replay(oracle, method, stl, scale, trans, want, tag, channel=True)
     compare_dirs(got, want)
+
+
+def test_binary_vtk_cli(oracle, obstacle, tmp_path):
+    """--vtk-format binary: the same fields, exact float32 values"""
+    from test_gpu_geom import read_binary_vtk
+
+    stl, scale, trans = obstacle
+    got = str(tmp_path / "gpu")
+    run_cli("fluidsim_legacy", case_args(stl, scale, trans, got, ["--vtk-format", "binary"]),
+            str(tmp_path))
+    want = str(tmp_path / "oracle")
+    replay(oracle, 0, stl, scale, trans, want, "legacy")
+    for step in range(0, STEPS + 1, EVERY):
+        _, rho = read_binary_vtk(os.path.join(got, f"rho_legacy_{step:06d}.vtk"), 1)
+        with open(os.path.join(want, f"rho_legacy_{step:06d}.vtk")) as fh:
+            text = fh.read().split("\n")[10:10 + rho.size]
+        # the ASCII file holds the same values printed as "%.6g"
+        assert ["%.6g" % float(x) for x in rho] == text
+        _, vel = read_binary_vtk(os.path.join(got, f"vel_legacy_{step:06d}.vtk"), 3)
+        assert vel.size == 3 * rho.size

@@ -119,3 +119,69 @@ def test_reinit_equilibrium_reference_entry(fs, oracle):
     o = oracle.Oracle(0, nx, ny, nz, 0.6)
     o.reinit(rho, ux, uy, uz)
     np.testing.assert_array_equal(f.to_host(np.float32, 19 * n), o.state())
+
+
+def test_field_diff_and_shift(fs):
+    """fs_diff / fs_shift and the reference-named launch_diff / launch_shift
+    (field_ops.cu:20-41): a - b and a + bias, elementwise in float32."""
+    import ctypes as C
+
+    from myfluidsimulation_amd import DeviceBuffer
+    from myfluidsimulation_amd.capi import call, lib
+
+    n = 70001  # not a multiple of the block size
+    rng = np.random.default_rng(3)
+    a, b = (rng.standard_normal(n).astype(np.float32) for _ in range(2))
+    ba, bb = DeviceBuffer.from_host(a), DeviceBuffer.from_host(b)
+    out = DeviceBuffer(4 * n)
+    call("fs_diff", C.c_void_p(ba.ptr), C.c_void_p(bb.ptr), C.c_void_p(out.ptr), n)
+    np.testing.assert_array_equal(out.to_host(np.float32, n), a - b)
+    call("fs_shift", C.c_void_p(ba.ptr), C.c_float(-1.0), C.c_void_p(out.ptr), n)
+    np.testing.assert_array_equal(out.to_host(np.float32, n), a + np.float32(-1.0))
+    lib().launch_diff(C.c_void_p(bb.ptr), C.c_void_p(ba.ptr), C.c_void_p(out.ptr), n)
+    lib().fs_device_synchronize()
+    np.testing.assert_array_equal(out.to_host(np.float32, n), b - a)
+    lib().launch_shift(C.c_void_p(ba.ptr), C.c_float(0.25), C.c_void_p(out.ptr), n)
+    lib().fs_device_synchronize()
+    np.testing.assert_array_equal(out.to_host(np.float32, n), a + np.float32(0.25))
+
+
+def read_binary_vtk(path, ncomp):
+    data = open(path, "rb").read()
+    lines, pos = [], 0
+    n_header = 10 if ncomp == 1 else 9
+    for _ in range(n_header):
+        end = data.index(b"\n", pos)
+        lines.append(data[pos:end].decode())
+        pos = end + 1
+    n = int(lines[7].split()[1])
+    vals = np.frombuffer(data[pos:pos + 4 * n * ncomp], dtype=">f4")
+    assert data[pos + 4 * n * ncomp:] == b"\n"
+    return lines, vals.astype(np.float32)
+
+
+def test_binary_vtk_writer(fs, oracle, tmp_path):
+    """BINARY legacy VTK: the ASCII writer's header with BINARY, then the exact
+    float32 values big-endian (vectors interleaved per point)."""
+    import ctypes as C
+
+    from myfluidsimulation_amd import DeviceBuffer
+    from myfluidsimulation_amd.capi import call
+
+    nx, ny, nz = 7, 5, 3
+    n = nx * ny * nz
+    rng = np.random.default_rng(8)
+    u, v, w = (rng.standard_normal(n).astype(np.float32) for _ in range(3))
+    bufs = [DeviceBuffer.from_host(a) for a in (u, v, w)]
+    ps, pv = str(tmp_path / "s.vtk"), str(tmp_path / "v.vtk")
+    call("fs_write_scalar_vtk_binary", ps.encode(), C.c_void_p(bufs[0].ptr), nx, ny, nz)
+    call("fs_write_vector_vtk_binary", pv.encode(), *[C.c_void_p(b.ptr) for b in bufs], nx, ny, nz)
+    lines, vals = read_binary_vtk(ps, 1)
+    np.testing.assert_array_equal(vals, u)
+    ascii_path = str(tmp_path / "a.vtk")
+    oracle.write_scalar_vtk(ascii_path, u, nx, ny, nz)
+    ascii_head = open(ascii_path).read().split("\n")[:10]
+    assert lines == [ln.replace("ASCII", "BINARY") for ln in ascii_head]
+    lines, vals = read_binary_vtk(pv, 3)
+    np.testing.assert_array_equal(vals.reshape(n, 3), np.stack([u, v, w], axis=1))
+    assert lines[2] == "BINARY" and lines[7] == f"POINT_DATA {n}"
