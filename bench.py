@@ -14,6 +14,8 @@ Other workloads (--workload), the large single-domain roofline runs:
   channel768  768x256x256 channel, icosphere radius 40, Hybrid bands 1, 2, 4
               (and BGK, HOME)
   batch64     the 64-domain batched sweep against the same runs one by one
+  adjoint128  differentiable 128^3 Taylor-Green run (T = 2000, sqrt(T)
+              checkpoints): forward + backward MLUPS against plain steps
 The default channel line also carries a shorter run of the batched sweep.
 
 Timing: HIP events recorded on the solver's stream around the K steps
@@ -231,13 +233,65 @@ def batch_bench(steps, warmup, seed=7):
     return res
 
 
+def adjoint_bench(method, n, steps, segment=0):
+    """Differentiable run (the Taylor-Green inverse problem): forward with
+    sqrt(T) checkpoints, then the backward pass for dJ/du0 of
+    J = 1/2 |u(T) - u_target|^2, against T plain steps of the same solver.
+    Adjoint MLUPS = N T / (forward + recompute + backward wall time)."""
+    import ctypes as C
+
+    from myfluidsimulation_amd import DeviceBuffer, Solver
+    from myfluidsimulation_amd import workloads as wl
+    from myfluidsimulation_amd.solver import Adjoint
+
+    rho, ux, uy, uz = wl.taylor_green_3d(n, n, n, 0.05)
+    px, py, pz = wl.fourier_modes(n, n, n, 8, 0.01, seed=11)
+    s = Solver(method, n, n, n, 0.6)
+    s.reinit_from_macro(rho, ux, uy, uz)
+    s.step(2)
+    plain_ms = s.timed_steps(steps)
+    N = n ** 3
+    ins = [DeviceBuffer.from_host(a) for a in (rho, ux, uy, uz)]
+    outs = [DeviceBuffer(4 * N) for _ in range(4)]
+    drag = DeviceBuffer(4 * steps)
+    a = Adjoint(s, steps, segment)
+    ptr = lambda b: C.c_void_p(b.ptr) if b is not None else None  # noqa: E731
+    s.synchronize()
+    t0 = time.perf_counter()
+    a.forward_ptrs(*[ptr(b) for b in ins], None, *[ptr(b) for b in outs], ptr(drag))
+    s.synchronize()
+    t1 = time.perf_counter()
+    target = [DeviceBuffer.from_host(np.asarray(c, np.float32)) for c in (ux + px, uy + py, uz + pz)]
+    gu = [DeviceBuffer.from_host(outs[k + 1].to_host(np.float32, N) - target[k].to_host(np.float32, N))
+          for k in range(3)]
+    grads = [DeviceBuffer(4 * N) for _ in range(3)]
+    s.synchronize()
+    t2 = time.perf_counter()
+    a.backward_ptrs(None, *[ptr(b) for b in gu], None, None, *[ptr(b) for b in grads], None)
+    s.synchronize()
+    t3 = time.perf_counter()
+    g = np.concatenate([b.to_host(np.float32, N) for b in grads])
+    info = a.info()
+    a.close()
+    s.close()
+    wall = (t1 - t0) + (t3 - t2)
+    return {"method": method, "n": n, "steps": steps, "segment": info["segment"],
+            "checkpoints": info["checkpoints"], "device_GB": round(info["device_bytes"] / 1e9, 2),
+            "plain_MLUPS": round(N * steps / (plain_ms * 1e-3) / 1e6, 1),
+            "forward_s": round(t1 - t0, 4), "backward_s": round(t3 - t2, 4),
+            "adjoint_MLUPS": round(N * steps / wall / 1e6, 1),
+            "adjoint_over_plain": round((N * steps / wall) / (N * steps / (plain_ms * 1e-3)), 3),
+            "grad_finite": bool(np.isfinite(g).all()), "grad_norm": float(np.linalg.norm(g))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", default="channel",
-                    choices=["channel", "tgv256", "tgv512", "channel768", "batch64"])
+                    choices=["channel", "tgv256", "tgv512", "channel768", "batch64",
+                             "adjoint128"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the 64-domain batched sweep summary of the channel workload")
@@ -254,6 +308,19 @@ def main():
         print("no HIP device visible", file=sys.stderr)
         sys.exit(1)
 
+    if args.workload == "adjoint128":
+        steps = args.steps or 2000
+        runs = [adjoint_bench(m, 128, steps) for m in (0, 1, 2)]
+        head = runs[0]
+        print(json.dumps({
+            "metric": "adjoint forward+backward MLUPS vs plain forward (128^3, sqrt(T) checkpoints)",
+            "value": head["adjoint_MLUPS"], "unit": "MLUPS", "n_gpus": 1, "steps": steps,
+            "warmup": 2, "ms_per_step": round(1e3 * (head["forward_s"] + head["backward_s"]) / steps, 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (seeded)",
+            "config": {"workload": "Taylor-Green 128^3 inverse problem, dJ/du0, value = BGK"},
+            "methods": {r["method"]: r for r in runs}}))
+        return
     if args.workload == "batch64":
         print(json.dumps(batch_bench(args.steps or 2000, 20 if args.warmup is None
                                      else args.warmup)))

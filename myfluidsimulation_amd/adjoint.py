@@ -20,7 +20,7 @@ import ctypes as C
 
 import torch
 
-from .capi import call, lib
+from .solver import Adjoint
 
 
 def _ptr(t):
@@ -37,48 +37,6 @@ def _flat(t, n, name):
     if torch.cuda.is_available() and not t.is_cuda:
         raise ValueError(f"{name}: expected a GPU tensor")
     return t.detach().reshape(-1).contiguous()
-
-
-class Adjoint:
-    """Owner of one fs_adjoint handle (checkpoint memory of a T-step run)."""
-
-    def __init__(self, solver, steps: int, segment: int = 0, body=None):
-        import numpy as np
-
-        h = C.c_void_p()
-        b = None
-        if body is not None:
-            b = np.ascontiguousarray(np.asarray(body, np.uint8).reshape(-1))
-            if b.size != solver.n:
-                raise ValueError(f"body: expected {solver.n} bytes")
-        self._body = b
-        call("fs_adjoint_create", solver.h, int(steps), int(segment),
-             b.ctypes.data_as(C.c_void_p) if b is not None else None, C.byref(h))
-        self.h = h
-        self.solver = solver
-        self.steps = int(steps)
-
-    def info(self):
-        T, S, K, nb = C.c_int(), C.c_int(), C.c_int(), C.c_size_t()
-        call("fs_adjoint_info", self.h, C.byref(T), C.byref(S), C.byref(K), C.byref(nb))
-        return dict(steps=T.value, segment=S.value, checkpoints=K.value, device_bytes=nb.value)
-
-    def forward_ptrs(self, *ptrs):
-        call("fs_adjoint_forward", self.h, *ptrs)
-
-    def backward_ptrs(self, *ptrs):
-        call("fs_adjoint_backward", self.h, *ptrs)
-
-    def close(self):
-        if getattr(self, "h", None):
-            lib().fs_adjoint_destroy(self.h)
-            self.h = None
-
-    def __del__(self):
-        try:
-            self.close()
-        except Exception:
-            pass
 
 
 class _RunFn(torch.autograd.Function):

@@ -361,3 +361,46 @@ class Batch:
         out = np.empty(self._n(d) * (10 if self.methods[d] == HOME else 19), np.float32)
         call("fs_batch_copy_state", self.h, int(d), _host_ptr(out))
         return out
+
+
+class Adjoint:
+    """Owner of one fs_adjoint handle (checkpoint memory of a T-step run of
+    `solver`); forward_ptrs / backward_ptrs take the device pointers of
+    fs_adjoint_forward / fs_adjoint_backward.  The torch front end is
+    myfluidsimulation_amd.adjoint.DifferentiableRun."""
+
+    def __init__(self, solver, steps: int, segment: int = 0, body=None):
+        h = C.c_void_p()
+        b = None
+        if body is not None:
+            b = np.ascontiguousarray(np.asarray(body, np.uint8).reshape(-1))
+            if b.size != solver.n:
+                raise ValueError(f"body: expected {solver.n} bytes")
+        self._body = b
+        call("fs_adjoint_create", solver.h, int(steps), int(segment),
+             b.ctypes.data_as(C.c_void_p) if b is not None else None, C.byref(h))
+        self.h = h
+        self.solver = solver
+        self.steps = int(steps)
+
+    def info(self):
+        T, S, K, nb = C.c_int(), C.c_int(), C.c_int(), C.c_size_t()
+        call("fs_adjoint_info", self.h, C.byref(T), C.byref(S), C.byref(K), C.byref(nb))
+        return dict(steps=T.value, segment=S.value, checkpoints=K.value, device_bytes=nb.value)
+
+    def forward_ptrs(self, *ptrs):
+        call("fs_adjoint_forward", self.h, *ptrs)
+
+    def backward_ptrs(self, *ptrs):
+        call("fs_adjoint_backward", self.h, *ptrs)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fs_adjoint_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
