@@ -37,6 +37,7 @@ struct Dom {
   float* m[2] = {nullptr, nullptr};
   uint8_t* flags = nullptr;
   uint8_t* stage = nullptr;
+  uint32_t* nbm = nullptr;  // populations domains
   float* inlet = nullptr;
   std::vector<uint8_t> hflags;
 };
@@ -100,6 +101,7 @@ fs::PopLaunch pop_launch(const Dom& D) {
   fs::PopLaunch a;
   a.F = D.F;
   a.flags = D.flags;
+  a.nbm = D.nbm;
   a.rho = D.macro;
   a.ux = D.macro + D.g.P;
   a.uy = D.macro + 2 * D.g.P;
@@ -110,7 +112,7 @@ fs::PopLaunch pop_launch(const Dom& D) {
 
 int upload_flags(fs_batch* b, Dom& D) {
   FS_HIP(hipMemcpyAsync(D.stage, D.hflags.data(), D.g.n, hipMemcpyHostToDevice, b->stream));
-  FS_HIP(fs::launch_derive_flags(D.g, D.stage, D.flags, b->stream));
+  FS_HIP(fs::launch_derive_flags(D.g, D.stage, D.flags, D.nbm, b->stream));
   FS_HIP(hipStreamSynchronize(b->stream));
   return FS_OK;
 }
@@ -193,7 +195,7 @@ int fs_batch_create(const fs_domain_desc* descs, int ndomains, fs_batch** out) {
       bytes += (n + 255) & ~size_t(255);
       return at;
     };
-    std::vector<size_t> offs;  // per domain: F/m0, macro/m1, flags, stage, inlet
+    std::vector<size_t> offs;  // per domain: F/m0, macro/m1, flags, stage, inlet, nbm
     for (int d = 0; d < ndomains; ++d) {
       const fs_domain_desc& s = descs[d];
       if (s.method < 0 || s.method > 2) return bail(fail(FS_ERR_INVALID, "domain %d: method", d));
@@ -218,6 +220,7 @@ int fs_batch_create(const fs_domain_desc* descs, int ndomains, fs_batch** out) {
       offs.push_back(carve(P));
       offs.push_back(carve(P));
       offs.push_back(carve(sizeof(float) * size_t(s.ny) * s.nz));
+      offs.push_back(s.method == fs::kMethodHOME ? 0 : carve(sizeof(uint32_t) * P));
     }
     if (hipStreamCreate(&b->stream) != hipSuccess || hipStreamCreate(&b->side) != hipSuccess ||
         hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
@@ -228,7 +231,7 @@ int fs_batch_create(const fs_domain_desc* descs, int ndomains, fs_batch** out) {
     std::vector<uint16_t> popTile, homeTile;
     for (int d = 0; d < ndomains; ++d) {
       Dom& D = b->doms[size_t(d)];
-      const size_t* o = &offs[size_t(d) * 5];
+      const size_t* o = &offs[size_t(d) * 6];
       if (D.desc.method == fs::kMethodHOME) {
         D.m[0] = reinterpret_cast<float*>(b->arena + o[0]);
         D.m[1] = reinterpret_cast<float*>(b->arena + o[1]);
@@ -239,6 +242,7 @@ int fs_batch_create(const fs_domain_desc* descs, int ndomains, fs_batch** out) {
       D.flags = reinterpret_cast<uint8_t*>(b->arena + o[2]);
       D.stage = reinterpret_cast<uint8_t*>(b->arena + o[3]);
       D.inlet = reinterpret_cast<float*>(b->arena + o[4]);
+      if (D.desc.method != fs::kMethodHOME) D.nbm = reinterpret_cast<uint32_t*>(b->arena + o[5]);
       const float om = 1.0f / D.desc.tau;
       const fs::Phys ph = {om, om, om, D.desc.fx, D.desc.fy, D.desc.fz, FS_BOUNCE_HALFWAY};
       if (D.desc.method == fs::kMethodHOME) {

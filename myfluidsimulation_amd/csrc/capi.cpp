@@ -84,6 +84,7 @@ void free_solver(fs_solver* s) {
   (void)hipFree(s->mbuf[0]);
   (void)hipFree(s->mbuf[1]);
   (void)hipFree(s->flags);
+  (void)hipFree(s->nbm);
   (void)hipFree(s->stage);
   (void)hipFree(s->inlet);
   (void)hipFree(s->scratch);
@@ -94,7 +95,7 @@ void free_solver(fs_solver* s) {
 
 int upload_flags(fs_solver* s) {
   FS_HIP(hipMemcpyAsync(s->stage, s->hflags.data(), s->g.n, hipMemcpyHostToDevice, s->stream));
-  FS_HIP(fs::launch_derive_flags(s->g, s->stage, s->flags, s->stream));
+  FS_HIP(fs::launch_derive_flags(s->g, s->stage, s->flags, s->nbm, s->stream));
   FS_HIP(hipStreamSynchronize(s->stream));
   return FS_OK;
 }
@@ -140,6 +141,7 @@ fs::PopLaunch pop_args(fs_solver* s) {
   fs::PopLaunch a;
   a.F = s->F;
   a.flags = s->flags;
+  a.nbm = s->nbm;
   a.rho = s->macro;
   a.ux = s->macro + s->g.P;
   a.uy = s->macro + 2 * s->g.P;
@@ -254,6 +256,7 @@ int fs_solver_create(int method, int nx, int ny, int nz, float tau, float fx, fl
     } else {
       e = hipMalloc((void**)&s->F, sizeof(float) * 19 * P);
       if (e == hipSuccess) e = hipMalloc((void**)&s->macro, sizeof(float) * 4 * P);
+      if (e == hipSuccess) e = hipMalloc((void**)&s->nbm, sizeof(uint32_t) * P);
     }
     if (e == hipSuccess) e = hipMalloc((void**)&s->flags, P);
     if (e == hipSuccess) e = hipMalloc((void**)&s->stage, P);

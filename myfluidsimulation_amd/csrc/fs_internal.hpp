@@ -104,6 +104,7 @@ struct Phys {
 struct PopLaunch {
   float* F;
   const uint8_t* flags;
+  const uint32_t* nbm;  // bit q: the periodic neighbour c + e_q is solid
   float *rho, *ux, *uy, *uz;
   const float* inlet;  // ny*nz inlet x-velocity
 };
@@ -117,7 +118,10 @@ hipError_t launch_pop_reinit(const Geo& g, const PopLaunch& a, int mode, const f
 // populations in the swapped (odd) AA representation -> reference order
 hipError_t launch_pop_canonical(const Geo& g, const float* F, const uint8_t* flags, bool swapped,
                                 float* out, hipStream_t st);
-hipError_t launch_derive_flags(const Geo& g, const uint8_t* in, uint8_t* out, hipStream_t st);
+// derived flag bytes; nbm (nullable): per cell, bit q set when the periodic
+// neighbour c + e_q is solid
+hipError_t launch_derive_flags(const Geo& g, const uint8_t* in, uint8_t* out, uint32_t* nbm,
+                               hipStream_t st);
 
 struct HomeLaunch {
   const float* m;

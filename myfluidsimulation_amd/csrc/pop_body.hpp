@@ -62,15 +62,15 @@ __device__ __forceinline__ void inlet_emit(float (&f)[19], float uin) {
   });
 }
 
-// odd-step emission from origin cell `c` with neighbour offsets `o`
+// odd-step emission from origin cell `c` with neighbour offsets `o`; solid:
+// the cell's solid-neighbour mask (bit q: c + e_q is solid)
 template <bool CHECK>
-__device__ __forceinline__ void emit_odd(float* __restrict__ F, size_t P,
-                                         const uint8_t* __restrict__ flags, uint32_t c,
-                                         const Nbr& o, const float (&f)[19]) {
+__device__ __forceinline__ void emit_odd(float* __restrict__ F, size_t P, uint32_t solid,
+                                         uint32_t c, const Nbr& o, const float (&f)[19]) {
   for_q<0, 19>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
     const uint32_t d = c + nb_off<CX[q], CY[q], CZ[q]>(o);
-    if (CHECK && q != 0 && (flags[d] & F_SOLID)) {
+    if (CHECK && q != 0 && ((solid >> q) & 1u)) {
       st_pop(&F[size_t(OPP[q]) * P + c], f[q]);
     } else {
       st_pop(&F[size_t(q) * P + d], f[q]);
@@ -122,10 +122,12 @@ __device__ __forceinline__ void pop_cell(const Geo& g, const Phys& ph, const Pop
         f[q] = ld_pop(&F[size_t(OPP[q]) * P + s]);
       });
     } else {
+      const uint32_t solid = a.nbm[c];
       for_q<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         const uint32_t s = c + nb_off<-CX[q], -CY[q], -CZ[q]>(o);
-        if (q != 0 && (a.flags[s] & F_SOLID))
+        // c - e_q is the neighbour along e_opp(q)
+        if (q != 0 && ((solid >> OPP[q]) & 1u))
           f[q] = ld_pop(&F[size_t(q) * P + c]);
         else
           f[q] = ld_pop(&F[size_t(OPP[q]) * P + s]);
@@ -147,15 +149,15 @@ __device__ __forceinline__ void pop_cell(const Geo& g, const Phys& ph, const Pop
     }
   } else {
     if (!anyNear)
-      emit_odd<false>(F, P, a.flags, c, o, f);
+      emit_odd<false>(F, P, 0u, c, o, f);
     else
-      emit_odd<true>(F, P, a.flags, c, o, f);
+      emit_odd<true>(F, P, a.nbm[c], c, o, f);
     if (fl & F_FEEDER) {
       // the outlet cell (x+1 = nx-1) collides exactly like its feeder
       Nbr o1 = o;
       o1.xm = -1;
       o1.xp = -(g.nx - 1);
-      emit_odd<true>(F, P, a.flags, c + 1, o1, f);
+      emit_odd<true>(F, P, a.nbm[c + 1], c + 1, o1, f);
     }
   }
   if (MACRO) {

@@ -33,6 +33,7 @@ struct fs_solver {
   int home_variant = -1;  // HOME kernel variant, -1 = default for the grid
   // flags
   uint8_t* flags = nullptr;  // derived flag byte used by the kernels
+  uint32_t* nbm = nullptr;   // solid-neighbour masks (populations methods)
   uint8_t* stage = nullptr;  // public flag bytes uploaded from the host
   std::vector<uint8_t> hflags;
   float* inlet = nullptr;  // ny*nz
