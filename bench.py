@@ -19,13 +19,16 @@ Other workloads (--workload), the large single-domain roofline runs:
 The default channel line also carries a shorter run of the batched sweep.
 
 Timing: HIP events recorded on the solver's stream around the K steps
-(fs_solver_timed_steps); every step is one kernel launch, so the event time
-divided by K is the step kernel's average launch duration.
+(fs_solver_timed_steps).  HOME and the one-step BGK / Hybrid kernel launch
+once per step; the two-step BGK / Hybrid kernel (k_pop_pair, the default
+variant) launches once per two steps, so K and W are rounded up to even and
+the event time divided by K/2 is its average launch duration.
 
 roofline.achieved uses algorithmic bytes per lattice update (B/LUP): BGK and
-Hybrid 152 (19 floats read + 19 written, in place), HOME 80 (10 + 10),
-counted over fluid cells only (solid cells move no data); peak = 8.0 TB/s
-HBM3E.  MLUPS counts every cell, solid included.
+Hybrid 76 with the two-step kernel (19 floats read + 19 written per two
+updates), 152 with the one-step kernel, HOME 80 (10 + 10), counted over
+fluid cells only (solid cells move no data); peak = 8.0 TB/s HBM3E.  MLUPS
+counts every cell, solid included.
 
 cpu_baseline: the FP32 CPU oracle (scalar C++ restatement of the reference
 rule, OpenMP over z-planes, all host threads) timed on a bounded sample of
@@ -46,7 +49,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBPS = 8000.0
-B_LUP = {0: 152.0, 1: 80.0, 2: 152.0}
+B_LUP = {0: 152.0, 1: 80.0, 2: 152.0}  # one step per launch
+B_PAIR = 76.0  # BGK / Hybrid, two steps per launch
 NAMES = {0: "bgk", 1: "home", 2: "hybrid"}
 SEED = 2024
 
@@ -101,15 +105,25 @@ def run_method(case, method, steps, warmup, band=0):
     if case.inlet is not None:
         s.set_inlet_profile(case.inlet)
     s.reinit_from_macro(*case.fields)
+    variant = s.variant
+    pair = method != 1 and variant != 0
+    if pair:
+        steps, warmup = steps + steps % 2, warmup + warmup % 2
     if warmup:
         s.step(warmup)
     ms = s.timed_steps(steps)
     ok = bool(np.isfinite(s.macro(0)).all())
     s.close()
     per = ms / steps
-    gbps = case.fluid() * B_LUP[method] / (per * 1e-3) / 1e9
+    b_lup = B_PAIR if pair else B_LUP[method]
+    gbps = case.fluid() * b_lup / (per * 1e-3) / 1e9
+    steps_per_launch = 2 if pair else 1
+    kernel = ("k_pop_pair" if pair else "k_pop_step") if method != 1 else (
+        "k_home_tiled" if variant != 0 else "k_home_step")
     return {"MLUPS": round(case.n * steps / (ms * 1e-3) / 1e6, 1),
-            "ms_per_step": round(per, 5), "B_per_LUP": B_LUP[method],
+            "ms_per_step": round(per, 5), "B_per_LUP": b_lup, "variant": variant,
+            "kernel": kernel, "launch_ms": round(per * steps_per_launch, 5),
+            "bytes_per_launch": round(case.fluid() * b_lup * steps_per_launch),
             "achieved_GBps": round(gbps, 1), "roofline_frac": round(gbps / PEAK_HBM_GBPS, 4),
             "finite": ok}
 
@@ -363,7 +377,10 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "traffic_channel384.json")
     if args.workload == "channel" and os.path.exists(tpath):
         with open(tpath) as fh:
-            traffic = json.load(fh).get("bgk_bytes_per_launch")
+            prof = json.load(fh)
+        # only when the profile measured the kernel that ran here
+        if prof.get("kernel", "k_pop_step") == head["kernel"]:
+            traffic = prof.get("bgk_bytes_per_launch")
     line = {
         "metric": "MLUPS & % HBM roofline (BGK/HOME/hybrid); batched & adjoint MLUPS vs plain",
         "value": head["MLUPS"],
@@ -381,8 +398,8 @@ def main():
         "methods": methods,
         "roofline": {"bound": "hbm", "achieved": head["achieved_GBps"], "peak": PEAK_HBM_GBPS,
                      "unit": "GB/s", "frac": head["roofline_frac"], "traffic": traffic,
-                     "algorithmic_bytes_per_launch": round(case.fluid() * B_LUP[runs[0][1]]),
-                     "kernel": "k_pop_step (BGK, AA pattern)",
+                     "algorithmic_bytes_per_launch": head["bytes_per_launch"],
+                     "launch_ms": head["launch_ms"], "kernel": head["kernel"],
                      "measured_copy_ceiling_GBps": round(ceiling, 1)},
     }
     if args.workload == "channel" and not args.no_batch:

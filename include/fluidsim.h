@@ -142,8 +142,12 @@ int fs_solver_timed_steps(fs_solver* s, int nsteps, float* elapsed_ms);
 
 /* Extension: kernel variant selection.  HOME has a plain per-cell gather
  * (variant 0) and tiled kernels (1: 64x4 columns x 16 planes, 2: 64x8 x 16,
- * 3: 64x8 x 32); every variant gives the same bits.  -1 restores the
- * per-grid default.  BGK / Hybrid have one kernel (variant 0).
+ * 3: 64x8 x 32).  BGK / Hybrid have a one-step in-place kernel (variant 0)
+ * and kernels that advance two steps per launch through LDS (1: 64x4 x 16,
+ * 2: 64x8 x 16, 3: 64x8 x 32, 4: 64x4 x 32; a step(n) call with n odd runs
+ * one one-step launch).  Every variant gives the same bits.  -1 restores
+ * the per-grid default.  While an outlet cell sits at an x that is a
+ * multiple of 64 the solver runs variant 0 (get_variant reports it).
  * fs_solver_autotune times each applicable variant for trial_steps steps
  * from the current state (restored after each trial) and keeps the
  * fastest; *chosen (may be NULL) receives it. */

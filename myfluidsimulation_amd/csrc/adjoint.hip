@@ -116,7 +116,7 @@ __global__ void __launch_bounds__(256) k_adj_pop(Geo g, Phys ph, AdjPopLaunch a)
     // the outlet cell x+1 emits this cell's f* with its own neighbours
     Nbr o1 = o;
     o1.xm = -1;
-    o1.xp = -(g.nx - 1);
+    o1.xp = (cl.x + 2 == g.nx) ? -(g.nx - 1) : 1;
     float lx[19];
     adj_pull<true>(a.lin, P, a.flags, c + 1, o1, gd, lx);
 #pragma unroll
@@ -368,7 +368,7 @@ __global__ void __launch_bounds__(256) k_adj_home(Geo g, Phys ph, AdjHomeLaunch 
   if (fl & F_FEEDER) {
     Nbr o1 = o;
     o1.xm = -1;
-    o1.xp = -(g.nx - 1);
+    o1.xp = (cl.x + 2 == g.nx) ? -(g.nx - 1) : 1;
     Mom mo;
     load_mom10(a.m, P, s + 1, mo);
     float lo[10];

@@ -80,6 +80,7 @@ void free_solver(fs_solver* s) {
   if (!s) return;
   if (s->own) (void)hipStreamSynchronize(s->own);
   (void)hipFree(s->F);
+  (void)hipFree(s->F2);
   (void)hipFree(s->macro);
   (void)hipFree(s->mbuf[0]);
   (void)hipFree(s->mbuf[1]);
@@ -94,6 +95,17 @@ void free_solver(fs_solver* s) {
 
 
 int upload_flags(fs_solver* s) {
+  s->outlet_tile_start = false;
+  for (int z = 0; z < s->g.nz && !s->outlet_tile_start; ++z)
+    for (int y = 0; y < s->g.ny && !s->outlet_tile_start; ++y) {
+      const uint8_t* row = s->hflags.data() + (size_t(z) * s->g.ny + y) * s->g.nx;
+      for (int x = 0; x < s->g.nx; x += 64)
+        if (row[x] & FS_FLAG_OUTLET) {
+          s->outlet_tile_start = true;
+          break;
+        }
+    }
+  s->f2_synced = false;
   FS_HIP(hipMemcpyAsync(s->stage, s->hflags.data(), s->g.n, hipMemcpyHostToDevice, s->stream));
   FS_HIP(fs::launch_derive_flags(s->g, s->stage, s->flags, s->nbm, s->stream));
   FS_HIP(hipStreamSynchronize(s->stream));
@@ -130,6 +142,7 @@ int reset_impl(fs_solver* s) {
   } else {
     FS_HIP(fs::launch_pop_reset(s->g, pop_args(s), s->stream));
     s->parity = 0;
+    s->f2_synced = false;
   }
   FS_HIP(hipStreamSynchronize(s->stream));
   return FS_OK;
@@ -164,11 +177,23 @@ int solver_reinit_async(fs_solver* s, const float* rho, const float* ux, const f
       FS_HIP(fs::launch_pop_reinit(s->g, pop_args(s), 0, rho, ux, uy, uz, s->stream));
     }
     s->parity = 0;
+    s->f2_synced = false;
   } else {
     FS_HIP(fs::launch_pop_reinit(s->g, pop_args(s), 1, rho, ux, uy, uz, s->stream));
     s->parity = 0;
+    s->f2_synced = false;
   }
   return FS_OK;
+}
+
+static int solver_pop_variant(const fs_solver* s) {
+  if (s->outlet_tile_start) return fs::kPopAA;
+  return s->pop_variant >= 0 ? s->pop_variant : fs::pop_default_variant(s->g);
+}
+
+static int solver_home_variant(const fs_solver* s) {
+  if (s->outlet_tile_start) return fs::kHomePlain;
+  return s->home_variant >= 0 ? s->home_variant : fs::home_default_variant(s->g);
 }
 
 namespace {
@@ -176,17 +201,37 @@ namespace {
 int enqueue_steps(fs_solver* s, int nsteps) {
   if (nsteps < 0) return fail(FS_ERR_INVALID, "nsteps must be >= 0");
   if (s->method == fs::kMethodHOME) {
+    const int variant = solver_home_variant(s);
     for (int i = 0; i < nsteps; ++i) {
       fs::HomeLaunch a{s->mbuf[s->cur], s->mbuf[1 - s->cur], s->flags, s->inlet};
-      FS_HIP(fs::launch_home_step(s->g, s->ph, a, s->home_variant, s->stream));
+      FS_HIP(fs::launch_home_step(s->g, s->ph, a, variant, s->stream));
       s->cur = 1 - s->cur;
     }
   } else {
     const bool hyb = s->method == fs::kMethodHybrid;
-    const fs::PopLaunch a = pop_args(s);
-    for (int i = 0; i < nsteps; ++i) {
-      FS_HIP(fs::launch_pop_step(s->g, s->ph, a, s->parity != 0, hyb, i == nsteps - 1, s->stream));
-      s->parity ^= 1;
+    const int variant = solver_pop_variant(s);
+    int i = 0;
+    while (i < nsteps) {
+      if (variant != fs::kPopAA && s->parity == 0 && nsteps - i >= 2) {
+        // two steps F -> F2, then swap the buffers; F2's solid slots must
+        // hold F's frozen values, which nothing else writes
+        if (!s->f2_synced) {
+          FS_HIP(hipMemcpyAsync(s->F2, s->F, sizeof(float) * 19 * s->g.P,
+                                hipMemcpyDeviceToDevice, s->stream));
+          s->f2_synced = true;
+        }
+        const fs::PopLaunch pa = pop_args(s);
+        const fs::PairLaunch a{s->F, s->F2, pa.flags, pa.nbm, pa.rho, pa.ux, pa.uy, pa.uz,
+                               pa.inlet};
+        FS_HIP(fs::launch_pop_pair(s->g, s->ph, a, hyb, i + 2 == nsteps, variant, s->stream));
+        std::swap(s->F, s->F2);
+        i += 2;
+      } else {
+        FS_HIP(fs::launch_pop_step(s->g, s->ph, pop_args(s), s->parity != 0, hyb,
+                                   i == nsteps - 1, s->stream));
+        s->parity ^= 1;
+        i += 1;
+      }
     }
   }
   return FS_OK;
@@ -255,6 +300,7 @@ int fs_solver_create(int method, int nx, int ny, int nz, float tau, float fx, fl
       if (e == hipSuccess) e = hipMalloc((void**)&s->mbuf[1], sizeof(float) * 10 * P);
     } else {
       e = hipMalloc((void**)&s->F, sizeof(float) * 19 * P);
+      if (e == hipSuccess) e = hipMalloc((void**)&s->F2, sizeof(float) * 19 * P);
       if (e == hipSuccess) e = hipMalloc((void**)&s->macro, sizeof(float) * 4 * P);
       if (e == hipSuccess) e = hipMalloc((void**)&s->nbm, sizeof(uint32_t) * P);
     }
@@ -445,7 +491,12 @@ int fs_solver_timed_steps(fs_solver* s, int nsteps, float* elapsed_ms) {
 
 int fs_solver_set_variant(fs_solver* s, int variant) {
   if (int rc = check_solver(s)) return rc;
-  if (s->method != fs::kMethodHOME) return variant <= 0 ? FS_OK : fail(FS_ERR_INVALID, "no such variant");
+  if (s->method != fs::kMethodHOME) {
+    if (variant < -1 || variant >= fs::kPopVariants)
+      return fail(FS_ERR_INVALID, "BGK / Hybrid variant %d out of range", variant);
+    s->pop_variant = variant;
+    return FS_OK;
+  }
   if (variant >= 0 && !fs::home_variant_ok(s->g, variant))
     return fail(FS_ERR_INVALID, "HOME variant %d does not apply to this grid", variant);
   s->home_variant = variant;
@@ -455,47 +506,73 @@ int fs_solver_set_variant(fs_solver* s, int variant) {
 int fs_solver_get_variant(const fs_solver* s, int* variant) {
   if (int rc = check_solver(s)) return rc;
   if (!variant) return fail(FS_ERR_INVALID, "null output");
-  *variant = s->method == fs::kMethodHOME
-                 ? (s->home_variant >= 0 ? s->home_variant : fs::home_default_variant(s->g))
-                 : 0;
+  *variant = s->method == fs::kMethodHOME ? solver_home_variant(s) : solver_pop_variant(s);
   return FS_OK;
 }
 
 int fs_solver_autotune(fs_solver* s, int trial_steps, int* chosen) {
   if (int rc = check_solver(s)) return rc;
   if (trial_steps < 1) return fail(FS_ERR_INVALID, "trial_steps must be >= 1");
-  if (s->method != fs::kMethodHOME) {
+  const bool home = s->method == fs::kMethodHOME;
+  const int nvar = home ? fs::kHomeVariants : fs::kPopVariants;
+  // the tiled kernels need every outlet away from a tile row start
+  if (s->outlet_tile_start) {
+    if (home)
+      s->home_variant = fs::kHomePlain;
+    else
+      s->pop_variant = fs::kPopAA;
     if (chosen) *chosen = 0;
     return FS_OK;
   }
   // time every applicable variant from the current state, restoring it after
-  // each trial, and keep the fastest
-  const size_t bytes = sizeof(float) * 10 * s->g.P;
+  // each trial, and keep the fastest; population trials run an even number
+  // of steps from the natural representation
+  const size_t bytes = sizeof(float) * (home ? 10 : 19) * s->g.P;
   float* saved = nullptr;
   FS_HIP(hipMalloc((void**)&saved, bytes));
-  const int cur0 = s->cur;
+  const int cur0 = s->cur, parity0 = s->parity;
+  float* const F0 = s->F;
+  float* const F20 = s->F2;
+  const int steps = home ? trial_steps : 2 * ((trial_steps + 1) / 2);
   int best = -1;
   float best_ms = 0.0f;
   int rc = FS_OK;
-  hipError_t e = hipMemcpyAsync(saved, s->mbuf[cur0], bytes, hipMemcpyDeviceToDevice, s->stream);
-  for (int v = 0; v < fs::kHomeVariants && e == hipSuccess && rc == FS_OK; ++v) {
-    if (!fs::home_variant_ok(s->g, v)) continue;
-    s->home_variant = v;
+  hipError_t e = hipMemcpyAsync(saved, home ? s->mbuf[cur0] : s->F, bytes,
+                                hipMemcpyDeviceToDevice, s->stream);
+  auto restore = [&]() {
+    if (home) {
+      s->cur = cur0;
+      return hipMemcpyAsync(s->mbuf[cur0], saved, bytes, hipMemcpyDeviceToDevice, s->stream);
+    }
+    s->F = F0;
+    s->F2 = F20;
+    s->parity = parity0;
+    return hipMemcpyAsync(s->F, saved, bytes, hipMemcpyDeviceToDevice, s->stream);
+  };
+  for (int v = 0; v < nvar && e == hipSuccess && rc == FS_OK; ++v) {
+    if (home && !fs::home_variant_ok(s->g, v)) continue;
+    if (home)
+      s->home_variant = v;
+    else
+      s->pop_variant = v;
     float ms = 0.0f;
-    rc = enqueue_steps(s, 1);  // warm-up (code load, caches)
-    if (rc == FS_OK) rc = fs_solver_timed_steps(s, trial_steps, &ms);
+    // warm-up (code load, caches); a swapped AA state first takes one step
+    rc = enqueue_steps(s, home ? 1 : (s->parity ? 3 : 2));
+    if (rc == FS_OK) rc = fs_solver_timed_steps(s, steps, &ms);
     if (rc == FS_OK && (best < 0 || ms < best_ms)) {
       best = v;
       best_ms = ms;
     }
-    e = hipMemcpyAsync(s->mbuf[cur0], saved, bytes, hipMemcpyDeviceToDevice, s->stream);
-    s->cur = cur0;
+    e = restore();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
   (void)hipFree(saved);
   if (rc != FS_OK) return rc;
   if (e != hipSuccess) return fail(FS_ERR_DEVICE, "autotune: %s", hipGetErrorString(e));
-  s->home_variant = best;
+  if (home)
+    s->home_variant = best;
+  else
+    s->pop_variant = best;
   if (chosen) *chosen = best;
   return FS_OK;
 }
@@ -555,6 +632,7 @@ int fs_solver_load_state(fs_solver* s, const float* host) {
       FS_HIP(hipMemcpyAsync(s->F + size_t(q) * P, host + size_t(q) * N, sizeof(float) * N,
                             hipMemcpyHostToDevice, s->stream));
     s->parity = 0;
+    s->f2_synced = false;
   }
   FS_HIP(hipStreamSynchronize(s->stream));
   return FS_OK;

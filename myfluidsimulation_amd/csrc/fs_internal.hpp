@@ -119,9 +119,30 @@ hipError_t launch_pop_reinit(const Geo& g, const PopLaunch& a, int mode, const f
 hipError_t launch_pop_canonical(const Geo& g, const float* F, const uint8_t* flags, bool swapped,
                                 float* out, hipStream_t st);
 // derived flag bytes; nbm (nullable): per cell, bit q set when the periodic
-// neighbour c + e_q is solid
+// neighbour c + e_q is solid.  An outlet's derived F_BAND is its upstream
+// neighbour's (it collides as that cell).
 hipError_t launch_derive_flags(const Geo& g, const uint8_t* in, uint8_t* out, uint32_t* nbm,
                                hipStream_t st);
+
+// Two BGK / Hybrid steps per launch (lbm_pair.hip): Fin holds f^t in the
+// reference order, Fout receives f^{t+2}; the solid cells' slots of Fout
+// must already equal Fin's (the kernel never writes them).
+struct PairLaunch {
+  const float* Fin;
+  float* Fout;
+  const uint8_t* flags;
+  const uint32_t* nbm;
+  float *rho, *ux, *uy, *uz;
+  const float* inlet;
+};
+// BGK / Hybrid kernel variants: 0 the one-step AA kernel; 1..4 two steps per
+// launch on 64 x TY column tiles of ZC planes: (4,16), (8,16), (8,32), (4,32)
+constexpr int kPopAA = 0, kPairT4Z16 = 1, kPairT8Z16 = 2, kPairT8Z32 = 3, kPairT4Z32 = 4,
+              kPopVariants = 5;
+int pop_default_variant(const Geo& g);
+// variant in 1..kPopVariants-1 (others: the default)
+hipError_t launch_pop_pair(const Geo& g, const Phys& ph, const PairLaunch& a, bool hybrid,
+                           bool write_macro, int variant, hipStream_t st);
 
 struct HomeLaunch {
   const float* m;

@@ -121,6 +121,8 @@ __global__ void __launch_bounds__(256) k_derive_flags(Geo g, const uint8_t* __re
   if (mask) fl |= F_NEAR;
   if (nbm) nbm[c] = mask;
   if (cl.x + 1 < g.nx && (in[c + 1] & F_OUTLET)) fl |= F_FEEDER;
+  // an outlet (x >= 1) collides as its upstream neighbour, band bit included
+  if ((fl & F_OUTLET) && cl.x >= 1) fl = (fl & ~F_BAND) | (in[c - 1] & F_BAND);
   out[c] = fl;
 }
 

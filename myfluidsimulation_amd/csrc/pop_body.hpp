@@ -9,48 +9,70 @@ namespace fs {
 
 namespace {
 
+// Density and momentum of 19 populations, summed in q order.
+struct PopSums {
+  float r, jx, jy, jz;
+};
+__device__ __forceinline__ PopSums pop_sums(const float (&f)[19]) {
+  PopSums m{0.0f, 0.0f, 0.0f, 0.0f};
+  for_q<0, 19>([&](auto qc) {
+    constexpr int q = decltype(qc)::value;
+    m.r += f[q];
+    acc1<CX[q]>(m.jx, f[q]);
+    acc1<CY[q]>(m.jy, f[q]);
+    acc1<CZ[q]>(m.jz, f[q]);
+  });
+  return m;
+}
+
+// One cell's BGK / Hybrid collision (lbm3d_legacy.cu:78-100,
+// lbm3d_hybrid.cu:114-160): the moments and equilibrium terms once, then
+// post<q>(f_q) relaxes direction q, so a caller may finish only the
+// directions it needs.  u carries the forcing shift.
+template <bool HYB>
+struct PopCollision {
+  float r, ux, uy, uz, uu15;
+  WRho wr;
+  bool band;
+  float om, oh, osh;
+
+  __device__ __forceinline__ PopCollision(const PopSums& m, const Phys& ph, bool bandCell)
+      : r(m.r), wr(m.r), band(bandCell) {
+    ux = (m.r > 0.0f) ? m.jx / m.r : 0.0f;
+    uy = (m.r > 0.0f) ? m.jy / m.r : 0.0f;
+    uz = (m.r > 0.0f) ? m.jz / m.r : 0.0f;
+    ux += ph.fx;
+    uy += ph.fy;
+    uz += ph.fz;
+    const float uu = ux * ux + uy * uy + uz * uz;
+    uu15 = 1.5f * uu;
+    om = ph.omegaL;
+    oh = ph.omegaH;
+    osh = ph.omegaS - ph.omegaH;
+  }
+
+  template <int Q>
+  __device__ __forceinline__ float post(float fq) const {
+    const float fe = feq_q<Q>(wr, ux, uy, uz, uu15);
+    if (!HYB || band) return fq + om * (fe - fq);
+    const float ds = fq - fe;
+    return (fq + oh * (fe - fq)) + osh * (-ds);
+  }
+};
+
 // BGK / hybrid collision in place: f -> post-collision, returns moments.
 template <bool HYB>
 __device__ __forceinline__ void collide(float (&f)[19], float& r, float& ux, float& uy,
                                         float& uz, const Phys& ph, bool band) {
-  float rr = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
+  const PopCollision<HYB> pc(pop_sums(f), ph, band);
   for_q<0, 19>([&](auto qc) {
     constexpr int q = decltype(qc)::value;
-    rr += f[q];
-    acc1<CX[q]>(jx, f[q]);
-    acc1<CY[q]>(jy, f[q]);
-    acc1<CZ[q]>(jz, f[q]);
+    f[q] = pc.template post<q>(f[q]);
   });
-  float u0 = (rr > 0.0f) ? jx / rr : 0.0f;
-  float v0 = (rr > 0.0f) ? jy / rr : 0.0f;
-  float w0 = (rr > 0.0f) ? jz / rr : 0.0f;
-  u0 += ph.fx;
-  v0 += ph.fy;
-  w0 += ph.fz;
-  const float uu = u0 * u0 + v0 * v0 + w0 * w0;
-  const float uu15 = 1.5f * uu;
-  const WRho wr(rr);
-  if (!HYB || band) {
-    const float om = ph.omegaL;
-    for_q<0, 19>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      const float fe = feq_q<q>(wr, u0, v0, w0, uu15);
-      f[q] = f[q] + om * (fe - f[q]);
-    });
-  } else {
-    const float oh = ph.omegaH;
-    const float osh = ph.omegaS - ph.omegaH;
-    for_q<0, 19>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      const float fe = feq_q<q>(wr, u0, v0, w0, uu15);
-      const float ds = f[q] - fe;
-      f[q] = (f[q] + oh * (fe - f[q])) + osh * (-ds);
-    });
-  }
-  r = rr;
-  ux = u0;
-  uy = v0;
-  uz = w0;
+  r = pc.r;
+  ux = pc.ux;
+  uy = pc.uy;
+  uz = pc.uz;
 }
 
 __device__ __forceinline__ void inlet_emit(float (&f)[19], float uin) {
@@ -153,10 +175,11 @@ __device__ __forceinline__ void pop_cell(const Geo& g, const Phys& ph, const Pop
     else
       emit_odd<true>(F, P, a.nbm[c], c, o, f);
     if (fl & F_FEEDER) {
-      // the outlet cell (x+1 = nx-1) collides exactly like its feeder
+      // the outlet cell x+1 collides exactly like its feeder; its own x
+      // neighbours: the feeder, and x+2 (periodic when x+1 = nx-1)
       Nbr o1 = o;
       o1.xm = -1;
-      o1.xp = -(g.nx - 1);
+      o1.xp = (cl.x + 2 == g.nx) ? -(g.nx - 1) : 1;
       emit_odd<true>(F, P, a.nbm[c + 1], c + 1, o1, f);
     }
   }

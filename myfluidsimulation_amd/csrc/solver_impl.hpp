@@ -24,7 +24,10 @@ struct fs_solver {
   hipStream_t own = nullptr;
   hipStream_t stream = nullptr;
   // populations (BGK / Hybrid)
-  float* F = nullptr;
+  float* F = nullptr;      // the current populations
+  float* F2 = nullptr;     // second buffer of the two-step kernels (swapped with F)
+  bool f2_synced = false;  // F2's solid-cell slots equal F's
+  int pop_variant = -1;    // kernel variant, -1 = default for the grid
   float* macro = nullptr;  // 4 planes: rho, ux, uy, uz
   int parity = 0;          // AA representation: 0 natural, 1 swapped
   // moments (HOME)
@@ -36,6 +39,9 @@ struct fs_solver {
   uint32_t* nbm = nullptr;   // solid-neighbour masks (populations methods)
   uint8_t* stage = nullptr;  // public flag bytes uploaded from the host
   std::vector<uint8_t> hflags;
+  // an outlet cell starts a 64-wide row (x % 64 == 0): the tiled kernels,
+  // which take an outlet's result from the lane at x-1, do not apply
+  bool outlet_tile_start = false;
   float* inlet = nullptr;  // ny*nz
   float* scratch = nullptr;  // 19 planes for state conversion (lazy)
 

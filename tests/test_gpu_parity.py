@@ -244,3 +244,122 @@ def test_home_variants_and_autotune(fs, oracle):
                 np.testing.assert_array_equal(s.state(), before)
             s.step(6)
             np.testing.assert_array_equal(s.state(), ref)
+
+
+def pop_case(nx, ny, nz, seed=5):
+    """walls, inlet, outlet, an obstacle block and a Hybrid band next to it"""
+    obst = np.zeros((nz, ny, nx), np.uint8)
+    obst[nz // 3:nz // 3 + 3, ny // 3:ny // 3 + 3, nx // 4:nx // 4 + 4] = 1
+    flags = wl.channel_flags(nx, ny, nz, obst.reshape(-1))
+    band = np.zeros((nz, ny, nx), np.uint8)
+    band[:, :, nx // 4 - 2:nx // 4 + 6] = 2
+    flags = flags | (band.reshape(-1) & ~((flags & 1) * 2).astype(np.uint8))
+    rng = np.random.default_rng(seed)
+    n = nx * ny * nz
+    fields = (np.float32(1) + np.float32(0.01) * rng.standard_normal(n, dtype=np.float32),
+              np.float32(0.04) + np.float32(0.01) * rng.standard_normal(n, dtype=np.float32),
+              np.float32(0.01) * rng.standard_normal(n, dtype=np.float32),
+              np.float32(0.01) * rng.standard_normal(n, dtype=np.float32))
+    inlet = np.linspace(0.02, 0.05, ny * nz).astype(np.float32)
+    return flags, fields, inlet
+
+
+@pytest.mark.parametrize("method", [0, 2], ids=["bgk", "hybrid"])
+def test_pop_variants_and_autotune(fs, oracle, method):
+    """Every BGK / Hybrid kernel variant (the one-step AA kernel and the
+    two-step tiles) gives the oracle's bits through step chunks of odd and even
+    length, so runs switch between the two kernels and both representations;
+    autotune leaves the state untouched and picks one of them."""
+    nx, ny, nz = 130, 20, 40
+    flags, fields, inlet = pop_case(nx, ny, nz)
+    chunks = (1, 4, 3, 2, 5)
+    o = oracle.Oracle(method, nx, ny, nz, 0.58, (2e-6, 0.0, 0.0))
+    o.set_flags(flags)
+    o.set_inlet(inlet)
+    if method == 2:
+        o.set_omegas(1.0 / 0.58, 1.3, 1.45)
+    o.reinit(*fields)
+    refs = []
+    for k in chunks:
+        o.step(k)
+        refs.append((o.state(), o.macros()))
+    for variant in (0, 1, 2, 3, 4, -1):
+        with fs.Solver(method, nx, ny, nz, 0.58, (2e-6, 0.0, 0.0)) as s:
+            s.set_flags(flags)
+            s.set_inlet_profile(inlet)
+            if method == 2:
+                s.set_omegas(1.0 / 0.58, 1.3, 1.45)
+            s.set_variant(variant)
+            s.reinit_from_macro(*fields)
+            if variant == -1:
+                before = s.state()
+                chosen = s.autotune(3)
+                assert chosen in (0, 1, 2, 3, 4) and s.variant == chosen
+                np.testing.assert_array_equal(s.state(), before)
+            else:
+                assert s.variant == variant
+            for k, (st, mac) in zip(chunks, refs):
+                s.step(k)
+                np.testing.assert_array_equal(s.state(), st, err_msg=f"variant {variant}")
+                for g, r in zip(s.macros(), mac):
+                    np.testing.assert_array_equal(g, r, err_msg=f"variant {variant}")
+
+
+@pytest.mark.parametrize("dims", [(130, 9, 17), (64, 8, 16), (70, 5, 1), (3, 2, 2), (200, 3, 33)])
+def test_pop_pair_tile_shapes(fs, oracle, dims):
+    """The two-step kernel on partial tiles and z-chunks, one-plane and tiny
+    grids (the ring cells wrap onto the tile itself)."""
+    nx, ny, nz = dims
+    n = nx * ny * nz
+    rng = np.random.default_rng(11)
+    fields = (np.float32(1) + np.float32(0.01) * rng.standard_normal(n, dtype=np.float32),
+              np.float32(0.02) * rng.standard_normal(n, dtype=np.float32),
+              np.float32(0.02) * rng.standard_normal(n, dtype=np.float32),
+              np.float32(0.02) * rng.standard_normal(n, dtype=np.float32))
+    flags = None
+    if nx >= 8 and ny >= 3:
+        f = np.zeros((nz, ny, nx), np.uint8)
+        f[:, 0, :] = 1
+        f[:, ny - 1, nx // 2:nx // 2 + 3] = 1
+        flags = f.reshape(-1)
+    for variant in (1, 2, 3, 4):
+        nxyz = (nx, ny, nz)
+        o = oracle.Oracle(0, *nxyz, 0.6, (1e-5, 0.0, 0.0))
+        s = fs.Solver(0, *nxyz, 0.6, (1e-5, 0.0, 0.0))
+        if flags is not None:
+            o.set_flags(flags)
+            s.set_flags(flags)
+        o.reinit(*fields)
+        s.reinit_from_macro(*fields)
+        s.set_variant(variant)
+        s.step(6)
+        o.step(6)
+        np.testing.assert_array_equal(s.state(), o.state(), err_msg=f"{dims} variant {variant}")
+        for g, r in zip(s.macros(), o.macros()):
+            np.testing.assert_array_equal(g, r)
+        s.close()
+        o.close()
+
+
+@pytest.mark.parametrize("method", [0, 1, 2], ids=["bgk", "home", "hybrid"])
+def test_outlet_at_tile_row_start(fs, oracle, method):
+    """An outlet column at x = 64 (upstream x = 63 in the previous 64-wide
+    row): the tiled kernels take an outlet's result from the lane at x-1, so
+    the solver runs its one-step kernels and still gives the oracle's bits."""
+    nx, ny, nz = 100, 6, 5
+    f = np.zeros((nz, ny, nx), np.uint8)
+    f[:, :, 0] = 4
+    f[:, :, 64] = 8
+    f[:, 0, :] = 1
+    f[:, 0, 0] = 1
+    f[:, 0, 64] = 1
+    inlet = np.full(ny * nz, 0.03, np.float32)
+    fields = wl.uniform_flow(nx * ny * nz, 0.03)
+    with fs.Solver(method, nx, ny, nz, 0.6) as s:
+        s.set_flags(f.reshape(-1))
+        assert s.variant == 0
+        s.set_variant(-1)
+        assert s.variant == 0
+    res = run_pair(fs, oracle, method, (nx, ny, nz), 0.6, 8, flags=f.reshape(-1), fields=fields,
+                   inlet=inlet)
+    assert_exact(res, "outlet at x=64")
