@@ -158,19 +158,20 @@ __global__ void __launch_bounds__(64 * TY) k_pop_pair(Geo g, Phys ph, PairLaunch
     const uint32_t curNbm = nNbm;
     // ---- step t on plane p: tile cells ----
     if (active && !(curFl & F_SOLID)) {
-      float f[19];
       if (curFl & F_INLET) {
+        float f[19];
         inlet_emit(f, a.inlet[size_t(pw) * g.ny + y]);
+        for_q<0, 19>([&](auto qc) {
+          constexpr int q = decltype(qc)::value;
+          lds[L::at(q, ly + 1, lx + 1)] = f[q];
+        });
       } else {
-        float r, u, v, w;
-#pragma unroll
-        for (int q = 0; q < 19; ++q) f[q] = nf[q];
-        collide<HYB>(f, r, u, v, w, ph, (curFl & F_BAND) != 0);
+        const PopCollision<HYB> pc(pop_sums(nf), ph, (curFl & F_BAND) != 0);
+        for_q<0, 19>([&](auto qc) {
+          constexpr int q = decltype(qc)::value;
+          lds[L::at(q, ly + 1, lx + 1)] = pc.template post<q>(nf[q]);
+        });
       }
-      for_q<0, 19>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        lds[L::at(q, ly + 1, lx + 1)] = f[q];
-      });
     }
     // ---- step t on plane p: ring cells ----
     if (side >= 0 && !(hcurFl & F_SOLID)) {
@@ -180,12 +181,10 @@ __global__ void __launch_bounds__(64 * TY) k_pop_pair(Geo g, Phys ph, PairLaunch
         inlet_emit(f, a.inlet[size_t(pw) * g.ny + gy]);
         ring_emit<TX, TY>(lds, side, row, col, [&](auto qc) { return f[decltype(qc)::value]; });
       } else {
-#pragma unroll
-        for (int q = 0; q < 19; ++q) f[q] = nh[q];
-        const PopCollision<HYB> pc(pop_sums(f), ph, (hcurFl & F_BAND) != 0);
+        const PopCollision<HYB> pc(pop_sums(nh), ph, (hcurFl & F_BAND) != 0);
         ring_emit<TX, TY>(lds, side, row, col, [&](auto qc) {
           constexpr int q = decltype(qc)::value;
-          return pc.template post<q>(f[q]);
+          return pc.template post<q>(nh[q]);
         });
       }
     }
@@ -197,11 +196,7 @@ __global__ void __launch_bounds__(64 * TY) k_pop_pair(Geo g, Phys ph, PairLaunch
     __syncthreads();
 
     // ---- gather: f^{t+1} for planes p+1 (A), p (B), p-1 (C) ----
-    float A[5], B[9], Cd[5], cOv[5];
-#pragma unroll
-    for (int k = 0; k < 5; ++k) A[k] = Cd[k] = cOv[k] = 0.0f;
-#pragma unroll
-    for (int k = 0; k < 9; ++k) B[k] = 0.0f;
+    float A[5], B[9], Cd[5];
     if (active) {
       for_q<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
@@ -210,21 +205,11 @@ __global__ void __launch_bounds__(64 * TY) k_pop_pair(Geo g, Phys ph, PairLaunch
         if constexpr (CZ[q] == 0) B[kSlot[q]] = v;
         if constexpr (CZ[q] < 0) Cd[kSlot[q]] = v;
       });
-    }
-    // bounce-back for the cell on plane p: where c - e_q is solid, f^{t+1}_q
-    // is the cell's own f*_{opp q}.  Its A values (gathered one plane ago)
-    // are patched now, its C values are kept until they are gathered.
-    const uint32_t bbm = (active && !(curFl & F_SOLID)) ? curNbm : 0u;
-    if (__ballot(bbm != 0) != 0) {
-      for_q<1, 19>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        if ((bbm >> OPP[q]) & 1u) {
-          const float v = lds[L::at(OPP[q], ly + 1, lx + 1)];
-          if constexpr (CZ[q] > 0) aPrev[kSlot[q]] = v;
-          if constexpr (CZ[q] == 0) B[kSlot[q]] = v;
-          if constexpr (CZ[q] < 0) cOv[kSlot[q]] = v;
-        }
-      });
+    } else {
+#pragma unroll
+      for (int k = 0; k < 5; ++k) A[k] = Cd[k] = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) B[k] = 0.0f;
     }
 
     // ---- step t+1 on plane o = p-1 ----
@@ -235,8 +220,6 @@ __global__ void __launch_bounds__(64 * TY) k_pop_pair(Geo g, Phys ph, PairLaunch
     const bool live = fin && !(fl & F_SOLID);
     const uint32_t co = uint32_t(ow * nxy + ownRow);
     float f[19];
-#pragma unroll
-    for (int q = 0; q < 19; ++q) f[q] = 0.0f;
     float r = 1.0f, u = 0.0f, v = 0.0f, w = 0.0f;
     if (live) {
       if (fl & F_INLET) {
@@ -251,6 +234,9 @@ __global__ void __launch_bounds__(64 * TY) k_pop_pair(Geo g, Phys ph, PairLaunch
         });
         collide<HYB>(f, r, u, v, w, ph, (fl & F_BAND) != 0);
       }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 19; ++q) f[q] = 0.0f;
     }
     // an outlet takes its upstream neighbour's result (the previous lane)
     const bool outlet = fin && (fl & F_OUTLET) != 0;
@@ -283,12 +269,28 @@ __global__ void __launch_bounds__(64 * TY) k_pop_pair(Geo g, Phys ph, PairLaunch
       a.uz[co] = w;
     }
 
+    // ---- bounce-back for the cell on plane p ----
+    // where c - e_q is solid, f^{t+1}_q is the cell's own f*_{opp q} (still in
+    // LDS): its A values (gathered one plane ago) and B values are patched
+    // now, its C values are kept until plane p is finished
+    const uint32_t bbm = (active && !(curFl & F_SOLID)) ? curNbm : 0u;
+    if (__ballot(bbm != 0) != 0) {
+      for_q<1, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if ((bbm >> OPP[q]) & 1u) {
+          const float val = lds[L::at(OPP[q], ly + 1, lx + 1)];
+          if constexpr (CZ[q] > 0) aPrev[kSlot[q]] = val;
+          if constexpr (CZ[q] == 0) B[kSlot[q]] = val;
+          if constexpr (CZ[q] < 0) cOvPrev[kSlot[q]] = val;
+        }
+      });
+    }
+
     // ---- advance the ring ----
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
       aPrev2[k] = aPrev[k];
       aPrev[k] = A[k];
-      cOvPrev[k] = cOv[k];
     }
 #pragma unroll
     for (int k = 0; k < 9; ++k) bPrev[k] = B[k];
