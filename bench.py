@@ -400,6 +400,8 @@ def main():
                      "unit": "GB/s", "frac": head["roofline_frac"], "traffic": traffic,
                      "algorithmic_bytes_per_launch": head["bytes_per_launch"],
                      "launch_ms": head["launch_ms"], "kernel": head["kernel"],
+                     "measured_traffic_GBps": (round(traffic / (head["launch_ms"] * 1e-3) / 1e9, 1)
+                                               if traffic else None),
                      "measured_copy_ceiling_GBps": round(ceiling, 1)},
     }
     if args.workload == "channel" and not args.no_batch:

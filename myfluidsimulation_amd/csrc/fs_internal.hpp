@@ -1,7 +1,9 @@
 // fs_internal.hpp -- definitions shared by the HIP kernels and the C ABI layer.
 //
 // Data layout in HBM (one solver):
-//   populations  F[q*P + c]   19 planes, one buffer, updated in place (AA)
+//   populations  F[q*P + c]   19 planes, two buffers: the two-step kernel reads
+//                             one and writes the other, the one-step AA kernel
+//                             updates one in place
 //   moments      m[k*P + c]   10 planes, two buffers (HOME ping-pong)
 //   macros       rho/ux/uy/uz separate planes of P floats (BGK / Hybrid)
 //   flags        uint8 per cell
