@@ -108,6 +108,11 @@ bool home_variant_ok(const Geo& g, int variant) {
 
 int home_default_variant(const Geo& g) {
   if (!home_variant_ok(g, kHomeT8Z32)) return kHomePlain;
+  // a workgroup walks its whole z-chunk: below 512 tiles of 64x4x16 (about
+  // 128^3, measured) too few of them fill the chip and the per-cell gather
+  // is faster
+  const long tiles4 = long((g.nx + 63) / 64) * ((g.ny + 3) / 4) * ((g.nz + 15) / 16);
+  if (tiles4 < 512) return kHomePlain;
   // deep chunks cut the re-computed halo planes, but only pay when they
   // still leave enough workgroups (>= 4 per CU) to fill the chip
   const long tiles8 = long((g.nx + 63) / 64) * ((g.ny + 7) / 8);

@@ -24,10 +24,12 @@ def rel_maxnorm(got, ref):
 
 
 def run_pair(fs, oracle, method, dims, tau, steps, flags=None, fields=None, inlet=None,
-             force=(0.0, 0.0, 0.0), substeps_split=None):
+             force=(0.0, 0.0, 0.0), substeps_split=None, variant=None):
     nx, ny, nz = dims
     n = nx * ny * nz
     s = fs.Solver(method, nx, ny, nz, tau, force)
+    if variant is not None:
+        s.set_variant(variant)
     o = oracle.Oracle(method, nx, ny, nz, tau, force)
     if flags is not None:
         s.set_flags(flags)
@@ -203,17 +205,22 @@ def test_load_state_roundtrip(fs, oracle):
 
 @pytest.mark.parametrize("dims", [(65, 12, 10), (130, 9, 17), (64, 8, 16)])
 def test_home_tile_shapes(fs, oracle, dims):
-    """HOME's tiled kernel on partial tiles and z-chunks, and the plain
-    gather it falls back to when nx-1 starts a 64-wide tile (nx = 65)."""
+    """HOME's tiled kernels on partial tiles and z-chunks, and the plain
+    gather the solver keeps when nx-1 starts a 64-wide tile (nx = 65)."""
     nx, ny, nz = dims
     obst = np.zeros((nz, ny, nx), np.uint8)
     obst[2:5, 3:6, 10:14] = 1
     flags = wl.channel_flags(nx, ny, nz, obst.reshape(-1))
     inlet = np.full(ny * nz, 0.04, np.float32)
     fields = wl.uniform_flow(nx * ny * nz, 0.04)
-    res = run_pair(fs, oracle, 1, (nx, ny, nz), 0.62, 9, flags=flags, fields=fields, inlet=inlet,
-                   force=(1e-6, 0.0, 0.0))
-    assert_exact(res, f"home tile {dims}")
+    tiled = (nx - 1) % 64 != 0
+    for variant in ((1, 2, 3) if tiled else (0,)):
+        res = run_pair(fs, oracle, 1, (nx, ny, nz), 0.62, 9, flags=flags, fields=fields,
+                       inlet=inlet, force=(1e-6, 0.0, 0.0), variant=variant)
+        assert_exact(res, f"home tile {dims} variant {variant}")
+    if not tiled:
+        with fs.Solver(1, nx, ny, nz, 0.62) as s, pytest.raises(fs.capi.FluidSimError):
+            s.set_variant(1)
 
 
 def test_home_variants_and_autotune(fs, oracle):
