@@ -7,9 +7,13 @@
 // over a work list of tiles (a tile belongs to one domain; the list maps
 // tile -> domain), so small domains no longer pay a launch each and large
 // and small ones share the chip.  The per-tile work is exactly the
-// single-domain kernels' (pop_body.hpp, home_body.hpp), so every domain's
-// fields equal a single-domain run bit for bit.
+// single-domain kernels' (pop_body.hpp, pair_body.hpp, home_body.hpp), so
+// every domain's fields equal a single-domain run bit for bit.  Population
+// domains advance two steps per launch (k_batch_pair, 32 x 8 tiles: the
+// sweep's edges of 32..128 cells waste fewer lanes than with 64-wide rows),
+// or one (k_batch_pop) for an odd step count.
 #include "home_body.hpp"
+#include "pair_body.hpp"
 #include "pop_body.hpp"
 
 namespace fs {
@@ -17,14 +21,17 @@ namespace fs {
 namespace {
 
 constexpr int kBatchHomeTX = 32, kBatchHomeTY = 8, kBatchHomeZC = 16;
+constexpr int kBatchPairTX = 32, kBatchPairTY = 8, kBatchPairZC = 16;
 
 template <bool ODD, bool MACRO>
 __global__ void __launch_bounds__(256) k_batch_pop(const BatchPopDomain* __restrict__ doms,
                                                    const uint16_t* __restrict__ tile_dom,
-                                                   uint32_t ntiles) {
+                                                   uint32_t ntiles, int cur) {
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const BatchPopDomain& D = doms[tile_dom[t]];
     const Geo g = D.g;
+    PopLaunch a = D.a;
+    if (cur) a.F = D.F1;
     const uint32_t local = t - D.tile0;
     const uint32_t z = local / D.tilesPerPlane;
     const uint32_t cp = (local - z * D.tilesPerPlane) * 256u + threadIdx.x;
@@ -36,9 +43,40 @@ __global__ void __launch_bounds__(256) k_batch_pop(const BatchPopDomain* __restr
     cl.x = int(cp - y * uint32_t(g.nx));
     cl.c = z * g.nxy + cp;
     if (D.hybrid)
-      pop_cell<ODD, true, MACRO>(g, D.ph, D.a, cl);
+      pop_cell<ODD, true, MACRO>(g, D.ph, a, cl);
     else
-      pop_cell<ODD, false, MACRO>(g, D.ph, D.a, cl);
+      pop_cell<ODD, false, MACRO>(g, D.ph, a, cl);
+  }
+}
+
+template <bool MACRO>
+__global__ void __launch_bounds__(kBatchPairTX * kBatchPairTY)
+    k_batch_pair(const BatchPopDomain* __restrict__ doms, const uint16_t* __restrict__ tile_dom,
+                 uint32_t ntiles, int cur) {
+  __shared__ float lds[19 * TileLds<kBatchPairTX, kBatchPairTY>::PL];
+  // every thread runs the same trip count: pair_tile synchronises the group
+  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const BatchPopDomain& D = doms[tile_dom[t]];
+    const uint32_t local = t - D.pairTile0;
+    const int bz = int(local % uint32_t(D.pairChunks));
+    const int rest = int(local / uint32_t(D.pairChunks));
+    PairLaunch a;
+    a.Fin = cur ? D.F1 : D.a.F;
+    a.Fout = cur ? D.a.F : D.F1;
+    a.flags = D.a.flags;
+    a.nbm = D.a.nbm;
+    a.rho = D.a.rho;
+    a.ux = D.a.ux;
+    a.uy = D.a.uy;
+    a.uz = D.a.uz;
+    a.inlet = D.a.inlet;
+    const int bx = rest % D.pairTilesX, by = rest / D.pairTilesX;
+    if (D.hybrid)
+      pair_tile<kBatchPairTX, kBatchPairTY, kBatchPairZC, true, MACRO>(D.g, D.ph, a, bx, by, bz,
+                                                                       lds);
+    else
+      pair_tile<kBatchPairTX, kBatchPairTY, kBatchPairZC, false, MACRO>(D.g, D.ph, a, bx, by, bz,
+                                                                        lds);
   }
 }
 
@@ -76,25 +114,45 @@ void batch_home_tiling(const Geo& g, int& tilesX, int& tilesY, int& chunks) {
 
 int batch_home_tile_width() { return kBatchHomeTX; }
 
+void batch_pair_tiling(const Geo& g, int& tilesX, int& tilesY, int& chunks) {
+  tilesX = (g.nx + kBatchPairTX - 1) / kBatchPairTX;
+  tilesY = (g.ny + kBatchPairTY - 1) / kBatchPairTY;
+  chunks = (g.nz + kBatchPairZC - 1) / kBatchPairZC;
+}
+
+int batch_pair_tile_width() { return kBatchPairTX; }
+
 hipError_t launch_batch_pop(const BatchPopDomain* doms, const uint16_t* tile_dom, uint32_t ntiles,
-                            bool odd, bool macro, int blocks, hipStream_t st) {
+                            int cur, bool odd, bool macro, int blocks, hipStream_t st) {
   if (ntiles == 0) return hipSuccess;
   const unsigned grid = unsigned(blocks <= 0 || uint32_t(blocks) > ntiles ? ntiles : blocks);
   if (odd) {
     if (macro)
       hipLaunchKernelGGL((k_batch_pop<true, true>), dim3(grid), dim3(256), 0, st, doms, tile_dom,
-                         ntiles);
+                         ntiles, cur);
     else
       hipLaunchKernelGGL((k_batch_pop<true, false>), dim3(grid), dim3(256), 0, st, doms, tile_dom,
-                         ntiles);
+                         ntiles, cur);
   } else {
     if (macro)
       hipLaunchKernelGGL((k_batch_pop<false, true>), dim3(grid), dim3(256), 0, st, doms, tile_dom,
-                         ntiles);
+                         ntiles, cur);
     else
       hipLaunchKernelGGL((k_batch_pop<false, false>), dim3(grid), dim3(256), 0, st, doms,
-                         tile_dom, ntiles);
+                         tile_dom, ntiles, cur);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_pair(const BatchPopDomain* doms, const uint16_t* tile_dom,
+                             uint32_t ntiles, int cur, bool macro, int blocks, hipStream_t st) {
+  if (ntiles == 0) return hipSuccess;
+  const unsigned grid = unsigned(blocks <= 0 || uint32_t(blocks) > ntiles ? ntiles : blocks);
+  const dim3 block(kBatchPairTX * kBatchPairTY);
+  if (macro)
+    hipLaunchKernelGGL(k_batch_pair<true>, dim3(grid), block, 0, st, doms, tile_dom, ntiles, cur);
+  else
+    hipLaunchKernelGGL(k_batch_pair<false>, dim3(grid), block, 0, st, doms, tile_dom, ntiles, cur);
   return hipGetLastError();
 }
 

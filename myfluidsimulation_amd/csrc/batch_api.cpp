@@ -3,6 +3,9 @@
 // A batch owns the device buffers of all its domains in one allocation, the
 // device descriptor tables and tile work lists read by batch.hip, and a
 // captured hipGraph of STEPS_PER_GRAPH steps that replays the step loop.
+// Population domains hold two population buffers and advance two steps per
+// launch (an odd step count ends with one in-place AA step); every domain
+// steps together, so one buffer index and one step parity describe them all.
 // The population domains and the HOME domains never read each other, so a
 // run of steps is two independent launch chains: populations on the batch's
 // stream, HOME on a second stream forked from it and joined back at the end,
@@ -32,7 +35,8 @@ struct Dom {
   fs::Geo g;
   int pop = -1;   // index in the populations table, or -1
   int home = -1;  // index in the HOME table, or -1
-  float* F = nullptr;
+  float* F = nullptr;   // population buffer 0
+  float* F1 = nullptr;  // population buffer 1
   float* macro = nullptr;
   float* m[2] = {nullptr, nullptr};
   uint8_t* flags = nullptr;
@@ -40,6 +44,7 @@ struct Dom {
   uint32_t* nbm = nullptr;  // populations domains
   float* inlet = nullptr;
   std::vector<uint8_t> hflags;
+  bool outletTileStart = false;  // population domain: an outlet starts a pair-tile row
 };
 
 }  // namespace
@@ -48,16 +53,19 @@ struct fs_batch {
   std::vector<Dom> doms;
   std::vector<fs::BatchPopDomain> popTab;
   std::vector<fs::BatchHomeDomain> homeTab;
-  uint32_t popTiles = 0, homeTiles = 0;
+  uint32_t popTiles = 0, homeTiles = 0, pairTiles = 0;
   char* arena = nullptr;
   fs::BatchPopDomain* d_pop = nullptr;
   fs::BatchHomeDomain* d_home = nullptr;
   uint16_t* d_popTile = nullptr;
+  uint16_t* d_pairTile = nullptr;
   uint16_t* d_homeTile = nullptr;
   hipStream_t stream = nullptr;
   hipStream_t side = nullptr;  // HOME chain
   hipEvent_t fork = nullptr, join = nullptr;
   int parity = 0;  // shared by every domain: they all step together
+  int popCur = 0;  // population buffer holding the state (0: F, 1: F1)
+  bool popPair = true;  // population domains take two steps per launch
   int popBlocks = 0, homeBlocks = 0;  // 0 = one workgroup per tile
   bool useGraph = true;
   hipGraph_t graph = nullptr;
@@ -82,6 +90,7 @@ void free_batch(fs_batch* b) {
   (void)hipFree(b->d_pop);
   (void)hipFree(b->d_home);
   (void)hipFree(b->d_popTile);
+  (void)hipFree(b->d_pairTile);
   (void)hipFree(b->d_homeTile);
   if (b->side) (void)hipStreamSynchronize(b->side);
   if (b->fork) (void)hipEventDestroy(b->fork);
@@ -110,9 +119,28 @@ fs::PopLaunch pop_launch(const Dom& D) {
   return a;
 }
 
+// the population launch arguments on the buffer that holds the state
+fs::PopLaunch cur_pop_launch(const fs_batch* b, const Dom& D) {
+  fs::PopLaunch a = pop_launch(D);
+  if (b->popCur) a.F = D.F1;
+  return a;
+}
+
+// The two-step kernel never writes solid cells: their frozen slots must be
+// equal in both population buffers after any write of the state or flags.
+int sync_pop_buffers(fs_batch* b, const Dom& D) {
+  if (D.desc.method == fs::kMethodHOME) return FS_OK;
+  const float* src = b->popCur ? D.F1 : D.F;
+  float* dst = b->popCur ? D.F : D.F1;
+  FS_HIP(hipMemcpyAsync(dst, src, sizeof(float) * 19 * D.g.P, hipMemcpyDeviceToDevice,
+                        b->stream));
+  return FS_OK;
+}
+
 int upload_flags(fs_batch* b, Dom& D) {
   FS_HIP(hipMemcpyAsync(D.stage, D.hflags.data(), D.g.n, hipMemcpyHostToDevice, b->stream));
   FS_HIP(fs::launch_derive_flags(D.g, D.stage, D.flags, D.nbm, b->stream));
+  if (int rc = sync_pop_buffers(b, D)) return rc;
   FS_HIP(hipStreamSynchronize(b->stream));
   return FS_OK;
 }
@@ -123,10 +151,20 @@ int launch_steps(fs_batch* b, int n, bool macro) {
   if (n <= 0) return FS_OK;
   FS_HIP(hipEventRecord(b->fork, b->stream));
   FS_HIP(hipStreamWaitEvent(b->side, b->fork, 0));
-  for (int i = 0; i < n; ++i) {
+  // populations: two steps per launch from the natural AA representation,
+  // one AA step otherwise (the step parity is the AA parity: pairs are even)
+  for (int i = 0; i < n;) {
     const bool odd = ((b->parity + i) & 1) != 0;
-    FS_HIP(fs::launch_batch_pop(b->d_pop, b->d_popTile, b->popTiles, odd, macro && i == n - 1,
-                                b->popBlocks, b->stream));
+    if (b->popPair && !odd && n - i >= 2) {
+      FS_HIP(fs::launch_batch_pair(b->d_pop, b->d_pairTile, b->pairTiles, b->popCur,
+                                   macro && i + 2 == n, b->popBlocks, b->stream));
+      b->popCur ^= 1;
+      i += 2;
+    } else {
+      FS_HIP(fs::launch_batch_pop(b->d_pop, b->d_popTile, b->popTiles, b->popCur, odd,
+                                  macro && i == n - 1, b->popBlocks, b->stream));
+      i += 1;
+    }
   }
   for (int i = 0; i < n; ++i)
     FS_HIP(fs::launch_batch_home(b->d_home, b->d_homeTile, b->homeTiles, (b->parity + i) & 1,
@@ -143,7 +181,10 @@ int build_graph(fs_batch* b) {
   const int rc = launch_steps(b, kStepsPerGraph, false);
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(b->stream, &g);
+  // capture ran nothing: the graph is replayed from (and returns to) parity
+  // 0 and population buffer 0
   b->parity = 0;
+  b->popCur = 0;
   if (rc != FS_OK) {
     if (g) (void)hipGraphDestroy(g);
     return rc;
@@ -159,7 +200,7 @@ int enqueue(fs_batch* b, int nsteps) {
   int left = nsteps;
   // whole graphs while more than one graph's worth remains (the last step
   // writes the BGK/Hybrid macros, so it is always a direct launch)
-  if (b->useGraph && b->parity == 0 && left > kStepsPerGraph) {
+  if (b->useGraph && b->parity == 0 && b->popCur == 0 && left > kStepsPerGraph) {
     if (!b->exec)
       if (int rc = build_graph(b)) return rc;
     while (left > kStepsPerGraph) {
@@ -195,7 +236,7 @@ int fs_batch_create(const fs_domain_desc* descs, int ndomains, fs_batch** out) {
       bytes += (n + 255) & ~size_t(255);
       return at;
     };
-    std::vector<size_t> offs;  // per domain: F/m0, macro/m1, flags, stage, inlet, nbm
+    std::vector<size_t> offs;  // per domain: F/m0, macro/m1, flags, stage, inlet, nbm, F1
     for (int d = 0; d < ndomains; ++d) {
       const fs_domain_desc& s = descs[d];
       if (s.method < 0 || s.method > 2) return bail(fail(FS_ERR_INVALID, "domain %d: method", d));
@@ -221,6 +262,7 @@ int fs_batch_create(const fs_domain_desc* descs, int ndomains, fs_batch** out) {
       offs.push_back(carve(P));
       offs.push_back(carve(sizeof(float) * size_t(s.ny) * s.nz));
       offs.push_back(s.method == fs::kMethodHOME ? 0 : carve(sizeof(uint32_t) * P));
+      offs.push_back(s.method == fs::kMethodHOME ? 0 : carve(sizeof(float) * 19 * P));
     }
     if (hipStreamCreate(&b->stream) != hipSuccess || hipStreamCreate(&b->side) != hipSuccess ||
         hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
@@ -228,10 +270,10 @@ int fs_batch_create(const fs_domain_desc* descs, int ndomains, fs_batch** out) {
       return bail(fail(FS_ERR_DEVICE, "stream"));
     if (hipMalloc((void**)&b->arena, bytes) != hipSuccess)
       return bail(fail(FS_ERR_ALLOC, "batch arena of %zu bytes", bytes));
-    std::vector<uint16_t> popTile, homeTile;
+    std::vector<uint16_t> popTile, pairTile, homeTile;
     for (int d = 0; d < ndomains; ++d) {
       Dom& D = b->doms[size_t(d)];
-      const size_t* o = &offs[size_t(d) * 6];
+      const size_t* o = &offs[size_t(d) * 7];
       if (D.desc.method == fs::kMethodHOME) {
         D.m[0] = reinterpret_cast<float*>(b->arena + o[0]);
         D.m[1] = reinterpret_cast<float*>(b->arena + o[1]);
@@ -242,7 +284,10 @@ int fs_batch_create(const fs_domain_desc* descs, int ndomains, fs_batch** out) {
       D.flags = reinterpret_cast<uint8_t*>(b->arena + o[2]);
       D.stage = reinterpret_cast<uint8_t*>(b->arena + o[3]);
       D.inlet = reinterpret_cast<float*>(b->arena + o[4]);
-      if (D.desc.method != fs::kMethodHOME) D.nbm = reinterpret_cast<uint32_t*>(b->arena + o[5]);
+      if (D.desc.method != fs::kMethodHOME) {
+        D.nbm = reinterpret_cast<uint32_t*>(b->arena + o[5]);
+        D.F1 = reinterpret_cast<float*>(b->arena + o[6]);
+      }
       const float om = 1.0f / D.desc.tau;
       const fs::Phys ph = {om, om, om, D.desc.fx, D.desc.fy, D.desc.fz, FS_BOUNCE_HALFWAY};
       if (D.desc.method == fs::kMethodHOME) {
@@ -266,26 +311,38 @@ int fs_batch_create(const fs_domain_desc* descs, int ndomains, fs_batch** out) {
         p.g = D.g;
         p.ph = ph;
         p.a = pop_launch(D);
+        p.F1 = D.F1;
         p.hybrid = D.desc.method == fs::kMethodHybrid;
         p.tile0 = uint32_t(popTile.size());
         p.tilesPerPlane = (D.g.nxy + 255) / 256;
+        p.pairTile0 = uint32_t(pairTile.size());
+        fs::batch_pair_tiling(D.g, p.pairTilesX, p.pairTilesY, p.pairChunks);
         D.pop = int(b->popTab.size());
         // work-list entries index the populations descriptor table
         popTile.insert(popTile.end(), size_t(p.tilesPerPlane) * D.g.nz, uint16_t(D.pop));
+        pairTile.insert(pairTile.end(),
+                        size_t(p.pairTilesX) * size_t(p.pairTilesY) * size_t(p.pairChunks),
+                        uint16_t(D.pop));
         b->popTab.push_back(p);
       }
     }
     b->popTiles = uint32_t(popTile.size());
+    b->pairTiles = uint32_t(pairTile.size());
     b->homeTiles = uint32_t(homeTile.size());
     // every work-list entry must name a descriptor of its own table
     for (uint16_t t : popTile)
       if (t >= b->popTab.size()) return bail(fail(FS_ERR_INVALID, "internal: pop work list"));
+    for (uint16_t t : pairTile)
+      if (t >= b->popTab.size()) return bail(fail(FS_ERR_INVALID, "internal: pair work list"));
     for (uint16_t t : homeTile)
       if (t >= b->homeTab.size()) return bail(fail(FS_ERR_INVALID, "internal: HOME work list"));
     for (size_t i = 0; i < b->popTab.size(); ++i) {
       const fs::BatchPopDomain& p = b->popTab[i];
       if (p.tile0 + uint64_t(p.tilesPerPlane) * p.g.nz > popTile.size() || popTile[p.tile0] != i)
         return bail(fail(FS_ERR_INVALID, "internal: pop tile range"));
+      const uint64_t np = uint64_t(p.pairTilesX) * p.pairTilesY * p.pairChunks;
+      if (p.pairTile0 + np > pairTile.size() || pairTile[p.pairTile0] != i)
+        return bail(fail(FS_ERR_INVALID, "internal: pair tile range"));
     }
     for (size_t i = 0; i < b->homeTab.size(); ++i) {
       const fs::BatchHomeDomain& h = b->homeTab[i];
@@ -298,6 +355,11 @@ int fs_batch_create(const fs_domain_desc* descs, int ndomains, fs_batch** out) {
       if (e == hipSuccess) e = hipMalloc((void**)&b->d_popTile, sizeof(uint16_t) * popTile.size());
       if (e == hipSuccess)
         e = hipMemcpy(b->d_popTile, popTile.data(), sizeof(uint16_t) * popTile.size(),
+                      hipMemcpyHostToDevice);
+      if (e == hipSuccess)
+        e = hipMalloc((void**)&b->d_pairTile, sizeof(uint16_t) * pairTile.size());
+      if (e == hipSuccess)
+        e = hipMemcpy(b->d_pairTile, pairTile.data(), sizeof(uint16_t) * pairTile.size(),
                       hipMemcpyHostToDevice);
     }
     if (e == hipSuccess && !b->homeTab.empty()) {
@@ -326,6 +388,7 @@ int fs_batch_create(const fs_domain_desc* descs, int ndomains, fs_batch** out) {
                                                 nullptr, b->stream)
                          : fs::launch_pop_reset(D.g, pop_launch(D), b->stream);
       if (r != hipSuccess) return bail(fail(FS_ERR_DEVICE, "reset: %s", hipGetErrorString(r)));
+      if (int rc = sync_pop_buffers(b, D)) return bail(rc);
     }
     if (hipStreamSynchronize(b->stream) != hipSuccess) return bail(fail(FS_ERR_DEVICE, "sync"));
     *out = b;
@@ -370,6 +433,25 @@ int fs_batch_set_flags(fs_batch* b, int d, const uint8_t* host_flags) {
         }
       }
     }
+  bool tileStart = false;
+  if (D.desc.method != fs::kMethodHOME)
+    for (int z = 0; z < g.nz && !tileStart; ++z)
+      for (int y = 0; y < g.ny && !tileStart; ++y) {
+        const uint8_t* row = host_flags + (size_t(z) * g.ny + y) * g.nx;
+        for (int x = 0; x < g.nx; x += fs::batch_pair_tile_width())
+          if (row[x] & FS_FLAG_OUTLET) tileStart = true;
+      }
+  FS_HIP(hipStreamSynchronize(b->stream));
+  D.outletTileStart = tileStart;
+  // the two-step kernel takes an outlet's result from the lane at x-1: while
+  // any population domain has an outlet that starts a tile row, all of them
+  // take one step per launch
+  bool pair = true;
+  for (const Dom& E : b->doms) pair = pair && !E.outletTileStart;
+  if (pair != b->popPair) {
+    b->popPair = pair;
+    drop_graph(b);
+  }
   for (size_t i = 0; i < g.n; ++i) D.hflags[i] = host_flags[i] & fs::F_PUBLIC;
   return upload_flags(b, D);
 }
@@ -395,11 +477,12 @@ int fs_batch_reinit_from_macro(fs_batch* b, int d, const float* rho, const float
   if (D.desc.method == fs::kMethodHOME) {
     FS_HIP(fs::launch_home_fill(D.g, D.m[0], D.m[1], rho, ux, uy, uz, b->stream));
   } else if (D.desc.method == fs::kMethodBGK && !rho) {
-    FS_HIP(fs::launch_pop_reset(D.g, pop_launch(D), b->stream));
+    FS_HIP(fs::launch_pop_reset(D.g, cur_pop_launch(b, D), b->stream));
   } else {
-    FS_HIP(fs::launch_pop_reinit(D.g, pop_launch(D), D.desc.method == fs::kMethodBGK ? 0 : 1, rho,
-                                 ux, uy, uz, b->stream));
+    FS_HIP(fs::launch_pop_reinit(D.g, cur_pop_launch(b, D), D.desc.method == fs::kMethodBGK ? 0 : 1,
+                                 rho, ux, uy, uz, b->stream));
   }
+  if (int rc = sync_pop_buffers(b, D)) return rc;
   FS_HIP(hipStreamSynchronize(b->stream));
   return FS_OK;
 }
@@ -466,10 +549,11 @@ int fs_batch_autotune(fs_batch* b, int trial_steps, int* pop_blocks, int* home_b
     bytes += sizeof(float) * (D.desc.method == fs::kMethodHOME ? 10 : 23) * D.g.P;
   char* saved = nullptr;
   FS_HIP(hipMalloc((void**)&saved, bytes));
+  const int cur0 = b->popCur;
   auto copy_state = [&](bool save) -> hipError_t {
     size_t at = 0;
     for (const Dom& D : b->doms) {
-      float* src = D.desc.method == fs::kMethodHOME ? D.m[0] : D.F;
+      float* src = D.desc.method == fs::kMethodHOME ? D.m[0] : (cur0 ? D.F1 : D.F);
       const size_t n = sizeof(float) * (D.desc.method == fs::kMethodHOME ? 10 : 19) * D.g.P;
       hipError_t e = save ? hipMemcpyAsync(saved + at, src, n, hipMemcpyDeviceToDevice, b->stream)
                           : hipMemcpyAsync(src, saved + at, n, hipMemcpyDeviceToDevice, b->stream);
@@ -505,6 +589,7 @@ int fs_batch_autotune(fs_batch* b, int trial_steps, int* pop_blocks, int* home_b
         bestPop = b->popBlocks;
         bestHome = b->homeBlocks;
       }
+      b->popCur = cur0;
       if (rc == FS_OK) e = copy_state(false);
       b->parity = 0;
     }
@@ -549,7 +634,8 @@ int fs_batch_copy_state(fs_batch* b, int d, float* host) {
   } else {
     float* tmp = nullptr;
     FS_HIP(hipMalloc((void**)&tmp, sizeof(float) * 19 * P));
-    hipError_t e = fs::launch_pop_canonical(D.g, D.F, D.flags, b->parity != 0, tmp, b->stream);
+    hipError_t e = fs::launch_pop_canonical(D.g, b->popCur ? D.F1 : D.F, D.flags, b->parity != 0,
+                                            tmp, b->stream);
     for (int q = 0; q < 19 && e == hipSuccess; ++q)
       e = hipMemcpyAsync(host + size_t(q) * N, tmp + size_t(q) * P, sizeof(float) * N,
                          hipMemcpyDeviceToHost, b->stream);

@@ -179,10 +179,13 @@ hipError_t launch_reinit_cs2(const float* rho, const float* ux, const float* uy,
 struct BatchPopDomain {
   Geo g;
   Phys ph;
-  PopLaunch a;
+  PopLaunch a;             // a.F: population buffer 0
+  float* F1;               // population buffer 1
   int hybrid;
-  uint32_t tile0;          // first work-list tile of this domain
+  uint32_t tile0;          // first one-step work-list tile of this domain
   uint32_t tilesPerPlane;  // 256-cell tiles per z-plane
+  uint32_t pairTile0;      // first two-step work-list tile
+  int pairTilesX, pairTilesY, pairChunks;
 };
 struct BatchHomeDomain {
   Geo g;
@@ -196,9 +199,15 @@ struct BatchHomeDomain {
 };
 void batch_home_tiling(const Geo& g, int& tilesX, int& tilesY, int& chunks);
 int batch_home_tile_width();
-// blocks <= 0: one workgroup per tile; otherwise a persistent grid of that size
+void batch_pair_tiling(const Geo& g, int& tilesX, int& tilesY, int& chunks);
+int batch_pair_tile_width();
+// blocks <= 0: one workgroup per tile; otherwise a persistent grid of that size.
+// cur: the population buffer (0 or 1) holding the state.
 hipError_t launch_batch_pop(const BatchPopDomain* doms, const uint16_t* tile_dom, uint32_t ntiles,
-                            bool odd, bool macro, int blocks, hipStream_t st);
+                            int cur, bool odd, bool macro, int blocks, hipStream_t st);
+// two steps of every population domain: buffer cur -> buffer 1-cur
+hipError_t launch_batch_pair(const BatchPopDomain* doms, const uint16_t* tile_dom,
+                             uint32_t ntiles, int cur, bool macro, int blocks, hipStream_t st);
 hipError_t launch_batch_home(const BatchHomeDomain* doms, const uint16_t* tile_dom,
                              uint32_t ntiles, int cur, int blocks, hipStream_t st);
 

@@ -115,3 +115,22 @@ def test_batch_rejects_bad_input(fs):
         b.step(1)
         with pytest.raises(FluidSimError):
             b.reinit_from_macro(0, np.ones(nx * ny * nz, np.float32))
+
+
+def test_batch_outlet_at_pair_tile_start(fs):
+    """A population outlet at x = 32 starts a 32-wide tile row of the two-step
+    batch kernel, so every population domain of the batch takes one step per
+    launch; results still equal the single-domain solvers'."""
+    doms = small_domains()[2:4]
+    nx, ny, nz = 33, 10, 10
+    doms.append(dict(method=0, dims=(nx, ny, nz), tau=0.6, flags=wl.channel_flags(nx, ny, nz),
+                     fields=wl.uniform_flow(nx * ny * nz, 0.03),
+                     inlet=np.full(ny * nz, 0.03, np.float32), force=(0.0, 0.0, 0.0)))
+    with make_batch(fs, doms) as b:
+        b.step(6)
+        b.step(5)
+        for i, d in enumerate(doms):
+            mac, st = single(fs, d, 11)
+            np.testing.assert_array_equal(b.state(i), st, err_msg=f"dom {i} state")
+            for k in range(4):
+                np.testing.assert_array_equal(b.macro(i, k), mac[k], err_msg=f"dom {i} macro {k}")
