@@ -144,10 +144,12 @@ int fs_solver_timed_steps(fs_solver* s, int nsteps, float* elapsed_ms);
  * (variant 0) and tiled kernels (1: 64x4 columns x 16 planes, 2: 64x8 x 16,
  * 3: 64x8 x 32).  BGK / Hybrid have a one-step in-place kernel (variant 0)
  * and kernels that advance two steps per launch through LDS (1: 64x4 x 16,
- * 2: 64x8 x 16, 3: 64x8 x 32, 4: 64x4 x 32; a step(n) call with n odd runs
- * one one-step launch).  Every variant gives the same bits.  -1 restores
- * the per-grid default.  While an outlet cell sits at an x that is a
- * multiple of 64 the solver runs variant 0 (get_variant reports it).
+ * 2: 64x8 x 16, 3: 64x8 x 32, 4: 64x4 x 32, 5: 32x8 x 16; a step(n) call
+ * with n odd runs one one-step launch).  Every variant gives the same bits.
+ * -1 restores the per-grid default (the one-step / plain kernels below about
+ * 128^3 cells).  While an outlet cell sits at an x that is a multiple of the
+ * variant's tile width (64, or 32 for variant 5) the solver runs variant 0
+ * (get_variant reports it).
  * fs_solver_autotune times each applicable variant for trial_steps steps
  * from the current state (restored after each trial) and keeps the
  * fastest; *chosen (may be NULL) receives it. */

@@ -95,14 +95,14 @@ void free_solver(fs_solver* s) {
 
 
 int upload_flags(fs_solver* s) {
-  s->outlet_tile_start = false;
-  for (int z = 0; z < s->g.nz && !s->outlet_tile_start; ++z)
-    for (int y = 0; y < s->g.ny && !s->outlet_tile_start; ++y) {
+  s->outlet_tile_start = s->outlet_tile_start32 = false;
+  for (int z = 0; z < s->g.nz; ++z)
+    for (int y = 0; y < s->g.ny; ++y) {
       const uint8_t* row = s->hflags.data() + (size_t(z) * s->g.ny + y) * s->g.nx;
-      for (int x = 0; x < s->g.nx; x += 64)
+      for (int x = 0; x < s->g.nx; x += 32)
         if (row[x] & FS_FLAG_OUTLET) {
-          s->outlet_tile_start = true;
-          break;
+          s->outlet_tile_start32 = true;
+          if (x % 64 == 0) s->outlet_tile_start = true;
         }
     }
   s->f2_synced = false;
@@ -187,8 +187,11 @@ int solver_reinit_async(fs_solver* s, const float* rho, const float* ux, const f
 }
 
 static int solver_pop_variant(const fs_solver* s) {
-  if (s->outlet_tile_start) return fs::kPopAA;
-  return s->pop_variant >= 0 ? s->pop_variant : fs::pop_default_variant(s->g);
+  const int v = s->pop_variant >= 0 ? s->pop_variant : fs::pop_default_variant(s->g);
+  if (v == fs::kPopAA) return v;
+  const bool blocked = fs::pop_variant_tile_width(v) == 32 ? s->outlet_tile_start32
+                                                            : s->outlet_tile_start;
+  return blocked ? fs::kPopAA : v;
 }
 
 static int solver_home_variant(const fs_solver* s) {

@@ -137,11 +137,14 @@ struct PairLaunch {
   float *rho, *ux, *uy, *uz;
   const float* inlet;
 };
-// BGK / Hybrid kernel variants: 0 the one-step AA kernel; 1..4 two steps per
-// launch on 64 x TY column tiles of ZC planes: (4,16), (8,16), (8,32), (4,32)
+// BGK / Hybrid kernel variants: 0 the one-step AA kernel; 1..5 two steps per
+// launch on TX x TY column tiles of ZC planes: 64 x (4,16), (8,16), (8,32),
+// (4,32), and 32 x (8,16)
 constexpr int kPopAA = 0, kPairT4Z16 = 1, kPairT8Z16 = 2, kPairT8Z32 = 3, kPairT4Z32 = 4,
-              kPopVariants = 5;
+              kPairX32T8Z16 = 5, kPopVariants = 6;
 int pop_default_variant(const Geo& g);
+// tile row width of a two-step variant: an outlet may not sit at a multiple
+int pop_variant_tile_width(int variant);
 // variant in 1..kPopVariants-1 (others: the default)
 hipError_t launch_pop_pair(const Geo& g, const Phys& ph, const PairLaunch& a, bool hybrid,
                            bool write_macro, int variant, hipStream_t st);

@@ -5,7 +5,8 @@
 // kernel reads f^t once and writes f^{t+2} once: 76 B per lattice update.
 // Step t's post-collision populations live only in LDS.
 //
-// A workgroup owns a 64 x TY column tile and a chunk of ZC planes and walks
+// A workgroup owns a TX x TY column tile (TX = 64 or 32) and a chunk of ZC
+// planes and walks
 // the planes z0-1 .. z0+ZC:
 //   1. every tile cell, and the ring of cells around the tile, reads its f^t
 //      (own slots, reference order) and collides (step t); the tile cells put
@@ -27,7 +28,7 @@
 // cell emits feq(1, (u_in, 0, 0)) whatever streams into it; an outlet cell
 // collides as its upstream neighbour x-1 (in step 1 it reads that cell's f^t,
 // in step 2 it takes the neighbour lane's result), so an outlet must not
-// start a 64-wide tile row (the caller checks).  Solid cells are never
+// start a tile row (the caller checks).  Solid cells are never
 // written: their frozen slots must already be equal in both buffers.
 #include "pair_body.hpp"
 
@@ -35,9 +36,8 @@ namespace fs {
 
 namespace {
 
-template <int TY, int ZC, bool HYB, bool MACRO>
-__global__ void __launch_bounds__(64 * TY) k_pop_pair(Geo g, Phys ph, PairLaunch a) {
-  constexpr int TX = 64;
+template <int TX, int TY, int ZC, bool HYB, bool MACRO>
+__global__ void __launch_bounds__(TX * TY) k_pop_pair(Geo g, Phys ph, PairLaunch a) {
   __shared__ float lds[19 * TileLds<TX, TY>::PL];
   // tile order: as k_home_tiled (x fastest, then y, then the z-chunk; each
   // XCD gets one contiguous run)
@@ -50,22 +50,25 @@ __global__ void __launch_bounds__(64 * TY) k_pop_pair(Geo g, Phys ph, PairLaunch
   pair_tile<TX, TY, ZC, HYB, MACRO>(g, ph, a, bx, rest % tilesY, rest / tilesY, lds);
 }
 
-template <int TY, int ZC>
+template <int TX, int TY, int ZC>
 hipError_t launch_pair(const Geo& g, const Phys& ph, const PairLaunch& a, bool hybrid, bool macro,
                        hipStream_t st) {
-  const unsigned blocks = unsigned(((g.nx + 63) / 64) * ((g.ny + TY - 1) / TY) * ((g.nz + ZC - 1) / ZC));
-  const dim3 grid(blocks), block(64 * TY);
+  const unsigned blocks =
+      unsigned(((g.nx + TX - 1) / TX) * ((g.ny + TY - 1) / TY) * ((g.nz + ZC - 1) / ZC));
+  const dim3 grid(blocks), block(TX * TY);
   const int key = (hybrid ? 2 : 0) | (macro ? 1 : 0);
   switch (key) {
-    case 0: hipLaunchKernelGGL((k_pop_pair<TY, ZC, false, false>), grid, block, 0, st, g, ph, a); break;
-    case 1: hipLaunchKernelGGL((k_pop_pair<TY, ZC, false, true>), grid, block, 0, st, g, ph, a); break;
-    case 2: hipLaunchKernelGGL((k_pop_pair<TY, ZC, true, false>), grid, block, 0, st, g, ph, a); break;
-    default: hipLaunchKernelGGL((k_pop_pair<TY, ZC, true, true>), grid, block, 0, st, g, ph, a); break;
+    case 0: hipLaunchKernelGGL((k_pop_pair<TX, TY, ZC, false, false>), grid, block, 0, st, g, ph, a); break;
+    case 1: hipLaunchKernelGGL((k_pop_pair<TX, TY, ZC, false, true>), grid, block, 0, st, g, ph, a); break;
+    case 2: hipLaunchKernelGGL((k_pop_pair<TX, TY, ZC, true, false>), grid, block, 0, st, g, ph, a); break;
+    default: hipLaunchKernelGGL((k_pop_pair<TX, TY, ZC, true, true>), grid, block, 0, st, g, ph, a); break;
   }
   return hipGetLastError();
 }
 
 }  // namespace
+
+int pop_variant_tile_width(int variant) { return variant == kPairX32T8Z16 ? 32 : 64; }
 
 int pop_default_variant(const Geo& g) {
   // A workgroup walks its whole z-chunk, so a small grid gives too few of
@@ -85,10 +88,11 @@ hipError_t launch_pop_pair(const Geo& g, const Phys& ph, const PairLaunch& a, bo
                            bool write_macro, int variant, hipStream_t st) {
   if (variant <= kPopAA || variant >= kPopVariants) variant = pop_default_variant(g);
   switch (variant) {
-    case kPairT8Z16: return launch_pair<8, 16>(g, ph, a, hybrid, write_macro, st);
-    case kPairT8Z32: return launch_pair<8, 32>(g, ph, a, hybrid, write_macro, st);
-    case kPairT4Z32: return launch_pair<4, 32>(g, ph, a, hybrid, write_macro, st);
-    default: return launch_pair<4, 16>(g, ph, a, hybrid, write_macro, st);
+    case kPairT8Z16: return launch_pair<64, 8, 16>(g, ph, a, hybrid, write_macro, st);
+    case kPairT8Z32: return launch_pair<64, 8, 32>(g, ph, a, hybrid, write_macro, st);
+    case kPairT4Z32: return launch_pair<64, 4, 32>(g, ph, a, hybrid, write_macro, st);
+    case kPairX32T8Z16: return launch_pair<32, 8, 16>(g, ph, a, hybrid, write_macro, st);
+    default: return launch_pair<64, 4, 16>(g, ph, a, hybrid, write_macro, st);
   }
 }
 

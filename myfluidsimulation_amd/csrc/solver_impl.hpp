@@ -39,9 +39,11 @@ struct fs_solver {
   uint32_t* nbm = nullptr;   // solid-neighbour masks (populations methods)
   uint8_t* stage = nullptr;  // public flag bytes uploaded from the host
   std::vector<uint8_t> hflags;
-  // an outlet cell starts a 64-wide row (x % 64 == 0): the tiled kernels,
-  // which take an outlet's result from the lane at x-1, do not apply
+  // an outlet cell starts a 64-wide (32-wide) row, x % 64 == 0 (x % 32 == 0):
+  // the tiled kernels of that width, which take an outlet's result from the
+  // lane at x-1, do not apply
   bool outlet_tile_start = false;
+  bool outlet_tile_start32 = false;
   float* inlet = nullptr;  // ny*nz
   float* scratch = nullptr;  // 19 planes for state conversion (lazy)
 

@@ -290,7 +290,7 @@ def test_pop_variants_and_autotune(fs, oracle, method):
     for k in chunks:
         o.step(k)
         refs.append((o.state(), o.macros()))
-    for variant in (0, 1, 2, 3, 4, -1):
+    for variant in (0, 1, 2, 3, 4, 5, -1):
         with fs.Solver(method, nx, ny, nz, 0.58, (2e-6, 0.0, 0.0)) as s:
             s.set_flags(flags)
             s.set_inlet_profile(inlet)
@@ -301,7 +301,7 @@ def test_pop_variants_and_autotune(fs, oracle, method):
             if variant == -1:
                 before = s.state()
                 chosen = s.autotune(3)
-                assert chosen in (0, 1, 2, 3, 4) and s.variant == chosen
+                assert chosen in (0, 1, 2, 3, 4, 5) and s.variant == chosen
                 np.testing.assert_array_equal(s.state(), before)
             else:
                 assert s.variant == variant
@@ -329,7 +329,7 @@ def test_pop_pair_tile_shapes(fs, oracle, dims):
         f[:, 0, :] = 1
         f[:, ny - 1, nx // 2:nx // 2 + 3] = 1
         flags = f.reshape(-1)
-    for variant in (1, 2, 3, 4):
+    for variant in (1, 2, 3, 4, 5):
         nxyz = (nx, ny, nz)
         o = oracle.Oracle(0, *nxyz, 0.6, (1e-5, 0.0, 0.0))
         s = fs.Solver(0, *nxyz, 0.6, (1e-5, 0.0, 0.0))
@@ -370,3 +370,26 @@ def test_outlet_at_tile_row_start(fs, oracle, method):
     res = run_pair(fs, oracle, method, (nx, ny, nz), 0.6, 8, flags=f.reshape(-1), fields=fields,
                    inlet=inlet)
     assert_exact(res, "outlet at x=64")
+
+
+@pytest.mark.parametrize("method", [0, 2], ids=["bgk", "hybrid"])
+def test_outlet_at_narrow_tile_row_start(fs, oracle, method):
+    """An outlet column at x = 96 starts a 32-wide row but not a 64-wide one:
+    the 32-wide two-step variant falls back to the one-step kernel, the
+    64-wide ones run; every choice gives the oracle's bits."""
+    nx, ny, nz = 130, 6, 5
+    f = np.zeros((nz, ny, nx), np.uint8)
+    f[:, :, 0] = 4
+    f[:, :, 96] = 8
+    f[:, 0, :] = 1
+    flags = f.reshape(-1)
+    inlet = np.full(ny * nz, 0.03, np.float32)
+    fields = wl.uniform_flow(nx * ny * nz, 0.03)
+    for variant, effective in ((5, 0), (1, 1), (2, 2)):
+        with fs.Solver(method, nx, ny, nz, 0.6) as s:
+            s.set_flags(flags)
+            s.set_variant(variant)
+            assert s.variant == effective
+        res = run_pair(fs, oracle, method, (nx, ny, nz), 0.6, 8, flags=flags, fields=fields,
+                       inlet=inlet, variant=variant)
+        assert_exact(res, f"outlet at x=96, variant {variant}")
