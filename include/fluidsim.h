@@ -50,7 +50,7 @@ extern "C" {
 #define FS_FLAG_SOLID 0x01u  /* solid (reference solid mask, lbm3d_legacy.cu:72) */
 #define FS_FLAG_BAND 0x02u   /* Hybrid BGK band (reference isLegacy, lbm3d_hybrid.cu:137) */
 #define FS_FLAG_INLET 0x04u  /* extension: equilibrium inlet cell */
-#define FS_FLAG_OUTLET 0x08u /* extension: zero-gradient outlet cell (x = nx-1) */
+#define FS_FLAG_OUTLET 0x08u /* extension: zero-gradient outlet cell (copies x-1) */
 
 /* ---- bounce-back partner rule (fs_solver_set_bounce_rule) ----
  * HALFWAY: a population hitting a solid cell returns in the opposite
