@@ -24,11 +24,14 @@ once per step; the two-step BGK / Hybrid kernel (k_pop_pair, the default
 variant) launches once per two steps, so K and W are rounded up to even and
 the event time divided by K/2 is its average launch duration.
 
-roofline.achieved uses algorithmic bytes per lattice update (B/LUP): BGK and
-Hybrid 76 with the two-step kernel (19 floats read + 19 written per two
-updates), 152 with the one-step kernel, HOME 80 (10 + 10), counted over
-fluid cells only (solid cells move no data); peak = 8.0 TB/s HBM3E.  MLUPS
-counts every cell, solid included.
+roofline.achieved = algorithmic bytes per launch / average launch time, with
+the algorithmic bytes of a lattice update fixed per method (BGK and Hybrid
+152 B: 19 floats read + 19 written; HOME 80 B: 10 + 10) times the lattice
+updates one launch performs (fluid cells only, solid cells move no data;
+two per cell for the two-step kernel); peak = 8.0 TB/s HBM3E.  The two-step
+kernel itself moves 76 B per update (kernel_B_per_LUP, kernel_GBps), and
+`traffic` / measured_traffic_GBps give what the counters saw.  MLUPS counts
+every cell, solid included.
 
 cpu_baseline: the FP32 CPU oracle (scalar C++ restatement of the reference
 rule, OpenMP over z-planes, all host threads) timed on a bounded sample of
@@ -50,7 +53,7 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBPS = 8000.0
 B_LUP = {0: 152.0, 1: 80.0, 2: 152.0}  # one step per launch
-B_PAIR = 76.0  # BGK / Hybrid, two steps per launch
+B_PAIR = 76.0  # bytes the two-step BGK / Hybrid kernel moves per update
 NAMES = {0: "bgk", 1: "home", 2: "hybrid"}
 SEED = 2024
 
@@ -115,7 +118,8 @@ def run_method(case, method, steps, warmup, band=0):
     ok = bool(np.isfinite(s.macro(0)).all())
     s.close()
     per = ms / steps
-    b_lup = B_PAIR if pair else B_LUP[method]
+    b_lup = B_LUP[method]
+    kernel_b = B_PAIR if pair else B_LUP[method]
     gbps = case.fluid() * b_lup / (per * 1e-3) / 1e9
     steps_per_launch = 2 if pair else 1
     kernel = ("k_pop_pair" if pair else "k_pop_step") if method != 1 else (
@@ -125,6 +129,8 @@ def run_method(case, method, steps, warmup, band=0):
             "kernel": kernel, "launch_ms": round(per * steps_per_launch, 5),
             "bytes_per_launch": round(case.fluid() * b_lup * steps_per_launch),
             "achieved_GBps": round(gbps, 1), "roofline_frac": round(gbps / PEAK_HBM_GBPS, 4),
+            "kernel_B_per_LUP": kernel_b,
+            "kernel_GBps": round(case.fluid() * kernel_b / (per * 1e-3) / 1e9, 1),
             "finite": ok}
 
 
@@ -400,6 +406,8 @@ def main():
                      "unit": "GB/s", "frac": head["roofline_frac"], "traffic": traffic,
                      "algorithmic_bytes_per_launch": head["bytes_per_launch"],
                      "launch_ms": head["launch_ms"], "kernel": head["kernel"],
+                     "kernel_B_per_LUP": head["kernel_B_per_LUP"],
+                     "kernel_GBps": head["kernel_GBps"],
                      "measured_traffic_GBps": (round(traffic / (head["launch_ms"] * 1e-3) / 1e9, 1)
                                                if traffic else None),
                      "measured_copy_ceiling_GBps": round(ceiling, 1)},
