@@ -78,6 +78,7 @@ __device__ __forceinline__ void pair_tile(const Geo& g, const Phys& ph, const Pa
   // from x-1.  Ring cells load when they collide: prefetching them too costs
   // registers that the tile loads need more.
   float nf[19], nh[19];
+  float nUin = 0.0f, hUin = 0.0f;  // inlet velocity of an inlet tile / ring cell
   uint32_t nNbm = 0;
   uint8_t fl1 = 0, fl2 = 0, hfl1 = 0, hfl2 = 0;
 #pragma unroll
@@ -92,17 +93,22 @@ __device__ __forceinline__ void pair_tile(const Geo& g, const Phys& ph, const Pa
     if (tile && active) {
       const uint32_t c = uint32_t(pw * nxy + ownRow);
       const uint32_t s = (f & F_OUTLET) ? c - 1 : c;
-      if (!(f & (F_SOLID | F_INLET))) {
+      if (!(f & F_SOLID)) {
 #pragma unroll
         for (int q = 0; q < 19; ++q) nf[q] = a.Fin[size_t(q) * P + s];
       }
+      // an inlet's velocity is loaded with the plane, not where it is used
+      if (f & F_INLET) nUin = a.inlet[size_t(pw) * g.ny + y];
       nNbm = (f & F_NEAR) ? a.nbm[c] : 0u;
     }
-    if (ring && side >= 0 && !(hf & (F_SOLID | F_INLET))) {
+    if (ring && side >= 0) {
       const uint32_t c = uint32_t(pw * nxy + ringRow);
       const uint32_t s = (hf & F_OUTLET) ? c - 1 : c;
+      if (!(hf & F_SOLID)) {
 #pragma unroll
-      for (int q = 0; q < 19; ++q) nh[q] = a.Fin[size_t(q) * P + s];
+        for (int q = 0; q < 19; ++q) nh[q] = a.Fin[size_t(q) * P + s];
+      }
+      if (hf & F_INLET) hUin = a.inlet[size_t(pw) * g.ny + gy];
     }
   };
   uint8_t curFl = 0, hcurFl = 0;
@@ -118,40 +124,56 @@ __device__ __forceinline__ void pair_tile(const Geo& g, const Phys& ph, const Pa
   for (int k = 0; k < 9; ++k) bPrev[k] = 0.0f;
   uint8_t flPrev = 0;
   uint32_t nbmPrev = 0;
+  float uinPrev = 0.0f;
 
   for (int p = z0 - 1; p <= z0 + ZCe; ++p) {
-    const int pw = wrap_plane(p, g.nz);
     load_pops(p, curFl, hcurFl, false, true);
     const uint32_t curNbm = nNbm;
+    const float curUin = nUin;
     // ---- step t on plane p: tile cells ----
+    // Inlet lanes run the fluid code on zero inputs with set_inlet moments;
+    // the choice is per wave, so waves without an inlet lane pay nothing.
+    const bool tileInlet = active && (curFl & F_INLET) != 0;
+    const bool tileInletWave = __ballot(tileInlet) != 0;
     if (active && !(curFl & F_SOLID)) {
-      if (curFl & F_INLET) {
-        float f[19];
-        inlet_emit(f, a.inlet[size_t(pw) * g.ny + y]);
-        for_q<0, 19>([&](auto qc) {
-          constexpr int q = decltype(qc)::value;
-          lds[L::at(q, ly + 1, lx + 1)] = f[q];
-        });
-      } else {
+      if (!tileInletWave) {
         const PopCollision<HYB> pc(pop_sums(nf), ph, (curFl & F_BAND) != 0);
         for_q<0, 19>([&](auto qc) {
           constexpr int q = decltype(qc)::value;
           lds[L::at(q, ly + 1, lx + 1)] = pc.template post<q>(nf[q]);
         });
+      } else {
+        float in[19];
+#pragma unroll
+        for (int q = 0; q < 19; ++q) in[q] = tileInlet ? 0.0f : nf[q];
+        PopCollision<HYB> pc(pop_sums(in), ph, (curFl & F_BAND) != 0);
+        if (tileInlet) pc.set_inlet(curUin);
+        for_q<0, 19>([&](auto qc) {
+          constexpr int q = decltype(qc)::value;
+          lds[L::at(q, ly + 1, lx + 1)] = pc.template post<q>(in[q]);
+        });
       }
     }
     // ---- step t on plane p: ring cells ----
+    const bool ringInlet = side >= 0 && (hcurFl & F_INLET) != 0;
+    const bool ringInletWave = __ballot(ringInlet) != 0;
     if (side >= 0 && !(hcurFl & F_SOLID)) {
       const int row = hy + 1, col = hx + 1;
-      float f[19];
-      if (hcurFl & F_INLET) {
-        inlet_emit(f, a.inlet[size_t(pw) * g.ny + gy]);
-        ring_emit<TX, TY>(lds, side, row, col, [&](auto qc) { return f[decltype(qc)::value]; });
-      } else {
+      if (!ringInletWave) {
         const PopCollision<HYB> pc(pop_sums(nh), ph, (hcurFl & F_BAND) != 0);
         ring_emit<TX, TY>(lds, side, row, col, [&](auto qc) {
           constexpr int q = decltype(qc)::value;
           return pc.template post<q>(nh[q]);
+        });
+      } else {
+        float in[19];
+#pragma unroll
+        for (int q = 0; q < 19; ++q) in[q] = ringInlet ? 0.0f : nh[q];
+        PopCollision<HYB> pc(pop_sums(in), ph, (hcurFl & F_BAND) != 0);
+        if (ringInlet) pc.set_inlet(hUin);
+        ring_emit<TX, TY>(lds, side, row, col, [&](auto qc) {
+          constexpr int q = decltype(qc)::value;
+          return pc.template post<q>(in[q]);
         });
       }
     }
@@ -188,24 +210,25 @@ __device__ __forceinline__ void pair_tile(const Geo& g, const Phys& ph, const Pa
     const uint32_t co = uint32_t(ow * nxy + ownRow);
     float f[19];
     float r = 1.0f, u = 0.0f, v = 0.0f, w = 0.0f;
+    const bool inl = live && (fl & F_INLET) != 0;
+    const bool anyInlet = __ballot(inl) != 0;
     if (live) {
-      if (fl & F_INLET) {
-        u = a.inlet[size_t(ow) * g.ny + y];
-        inlet_emit(f, u);
-      } else {
-        for_q<0, 19>([&](auto qc) {
-          constexpr int q = decltype(qc)::value;
-          if constexpr (CZ[q] > 0) f[q] = aPrev2[kSlot[q]];
-          if constexpr (CZ[q] == 0) f[q] = bPrev[kSlot[q]];
-          if constexpr (CZ[q] < 0) f[q] = ((nbmPrev >> OPP[q]) & 1u) ? cOvPrev[kSlot[q]] : Cd[kSlot[q]];
-        });
-        collide<HYB>(f, r, u, v, w, ph, (fl & F_BAND) != 0);
+      for_q<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (CZ[q] > 0) f[q] = aPrev2[kSlot[q]];
+        if constexpr (CZ[q] == 0) f[q] = bPrev[kSlot[q]];
+        if constexpr (CZ[q] < 0) f[q] = ((nbmPrev >> OPP[q]) & 1u) ? cOvPrev[kSlot[q]] : Cd[kSlot[q]];
+      });
+      if (anyInlet) {
+#pragma unroll
+        for (int q = 0; q < 19; ++q) f[q] = inl ? 0.0f : f[q];
       }
     } else {
 #pragma unroll
       for (int q = 0; q < 19; ++q) f[q] = 0.0f;
     }
-    // an outlet takes its upstream neighbour's result (the previous lane)
+    // an outlet collides its upstream neighbour's incoming populations (the
+    // previous lane's; its derived band bit is that cell's too)
     const bool outlet = fin && (fl & F_OUTLET) != 0;
     if (__ballot(outlet) != 0) {
 #pragma unroll
@@ -213,9 +236,28 @@ __device__ __forceinline__ void pair_tile(const Geo& g, const Phys& ph, const Pa
         const float left = __shfl_up(f[q], 1);
         if (outlet) f[q] = left;
       }
-      const float lr = __shfl_up(r, 1), lu = __shfl_up(u, 1);
-      const float lv = __shfl_up(v, 1), lw = __shfl_up(w, 1);
-      if (outlet) r = lr, u = lu, v = lv, w = lw;
+    }
+    if (live && !anyInlet) {
+      const PopCollision<HYB> pc(pop_sums(f), ph, (fl & F_BAND) != 0);
+      for_q<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        f[q] = pc.template post<q>(f[q]);
+      });
+      r = pc.r;
+      u = pc.ux;
+      v = pc.uy;
+      w = pc.uz;
+    } else if (live) {
+      PopCollision<HYB> pc(pop_sums(f), ph, (fl & F_BAND) != 0);
+      if (inl) pc.set_inlet(uinPrev);
+      for_q<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        f[q] = pc.template post<q>(f[q]);
+      });
+      r = pc.r;
+      u = pc.ux;
+      v = pc.uy;
+      w = pc.uz;
     }
     const uint32_t sm = live ? nbmPrev : 0u;
     const bool anyWall = __ballot(sm != 0) != 0;
@@ -263,6 +305,7 @@ __device__ __forceinline__ void pair_tile(const Geo& g, const Phys& ph, const Pa
     for (int k = 0; k < 9; ++k) bPrev[k] = B[k];
     flPrev = curFl;
     nbmPrev = curNbm;
+    uinPrev = curUin;
     curFl = nextFl;
     hcurFl = nextHfl;
     fl1 = fl2;

@@ -58,6 +58,22 @@ struct PopCollision {
     const float ds = fq - fe;
     return (fq + oh * (fe - fq)) + osh * (-ds);
   }
+
+  // An inlet cell emits feq(1, (u_in, 0, 0)) whatever streams into it
+  // (inlet_emit) and reports those moments.  With these moments and omega 1,
+  // post<q>(0.0f) = 0 + 1 * (feq - 0) rounds to exactly that feq (u.u and
+  // feq are inlet_emit's expressions), so inlet lanes can run the fluid
+  // lanes' code with zero inputs instead of a branch of their own.
+  __device__ __forceinline__ void set_inlet(float uin) {
+    r = 1.0f;
+    wr = WRho(1.0f);
+    ux = uin;
+    uy = 0.0f;
+    uz = 0.0f;
+    uu15 = 1.5f * (ux * ux + uy * uy + uz * uz);
+    om = 1.0f;
+    band = true;
+  }
 };
 
 // BGK / hybrid collision in place: f -> post-collision, returns moments.
