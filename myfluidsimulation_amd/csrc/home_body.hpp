@@ -178,6 +178,10 @@ __device__ __forceinline__ void home_tile(const Geo& g, const Phys& ph, const Ho
   for (int p = z0 - 1; p <= z0 + ZCe; ++p) {
     const Mom cur = nxtC;
     const uint8_t flCur = nxtFl;
+    // an inlet cell finished below (plane p-1) needs its velocity: the load
+    // is issued here, so the reconstruction and the barrier hide its latency
+    float uinFin = 0.0f;
+    if (active && (flPrev & F_INLET)) uinFin = a.inlet[size_t((p - 1 + g.nz) % g.nz) * g.ny + y];
     const Mom hm = nxtH;
     if (p < z0 + ZCe) load_plane(p + 1);
     if (active) {
@@ -230,7 +234,7 @@ __device__ __forceinline__ void home_tile(const Geo& g, const Phys& ph, const Ho
         out[0] = 1.0f;
       } else if (fl & F_INLET) {
         out[0] = 1.0f;
-        out[1] = a.inlet[size_t(ow) * g.ny + y];
+        out[1] = uinFin;
       } else {
         float f[19];
         for_q<0, 19>([&](auto qc) {

@@ -74,14 +74,10 @@ int pop_default_variant(const Geo& g) {
   // A workgroup walks its whole z-chunk, so a small grid gives too few of
   // them to fill 256 CUs: below 512 tiles of 64x4x16 (measured crossover,
   // about 128^3) the one-step kernel, with one workgroup per 256 cells, is
-  // faster.  64x8 tiles re-read fewer ring rows than 64x4 ones but hold one
-  // workgroup per CU: they pay when the grid gives many (>= 8 per CU).
-  const long cols = (g.nx + 63) / 64;
-  const long tiles4 = cols * ((g.ny + 3) / 4) * ((g.nz + 15) / 16);
-  if (tiles4 < 512) return kPopAA;
-  const long tiles8 = cols * ((g.ny + 7) / 8) * ((g.nz + 15) / 16);
-  if (tiles8 >= 2048) return kPairT8Z16;
-  return kPairT4Z16;
+  // faster.  Above it 64x4x16 tiles measured fastest on every workload
+  // (64x8 tiles re-read fewer halo rows but fit one workgroup per CU).
+  const long tiles4 = long((g.nx + 63) / 64) * ((g.ny + 3) / 4) * ((g.nz + 15) / 16);
+  return tiles4 < 512 ? kPopAA : kPairT4Z16;
 }
 
 hipError_t launch_pop_pair(const Geo& g, const Phys& ph, const PairLaunch& a, bool hybrid,
