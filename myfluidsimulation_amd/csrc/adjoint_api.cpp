@@ -235,8 +235,8 @@ int fs_adjoint_forward(fs_adjoint* a, const float* rho0, const float* ux0, const
     if (a->home) {
       if (drag)
         if (int rc = drag_step(a, s->mom_cur(), drag + n)) return rc;
-      const fs::HomeLaunch hl{s->mbuf[s->cur], s->mbuf[1 - s->cur], s->flags, s->inlet};
-      FS_HIP(fs::launch_home_step(g, s->ph, hl, s->home_variant, st));
+      const fs::HomeLaunch hl{s->mbuf[s->cur], s->mbuf[1 - s->cur], s->flags, s->nbm, s->inlet};
+      FS_HIP(fs::launch_home_step(g, s->ph, hl, solver_home_variant(s), st));
       s->cur = 1 - s->cur;
     } else {
       FS_HIP(fs::launch_pop_step(g, s->ph, pop_args(s), s->parity != 0, a->hyb, n == a->T - 1, st));
@@ -279,8 +279,8 @@ int fs_adjoint_backward(fs_adjoint* a, const float* g_rho, const float* g_ux, co
     if (a->home) {
       for (int n = n0; n < n1 - 1; ++n) {
         const fs::HomeLaunch hl{home_slot(a, k, n - n0), home_slot(a, k, n - n0 + 1), s->flags,
-                                s->inlet};
-        FS_HIP(fs::launch_home_step(g, s->ph, hl, s->home_variant, st));
+                                s->nbm, s->inlet};
+        FS_HIP(fs::launch_home_step(g, s->ph, hl, solver_home_variant(s), st));
       }
       for (int n = n1 - 1; n >= n0; --n) {
         fs::AdjHomeLaunch al;

@@ -94,6 +94,7 @@ __global__ void __launch_bounds__(kBatchHomeTX * kBatchHomeTY)
     a.m = cur ? D.m1 : D.m0;
     a.mn = cur ? D.m0 : D.m1;
     a.flags = D.flags;
+    a.nbm = D.nbm;
     a.inlet = D.inlet;
     if (D.ph.bounce == 0)
       home_tile<kBatchHomeTX, kBatchHomeTY, kBatchHomeZC, 0>(D.g, D.ph, a, rest % D.tilesX,

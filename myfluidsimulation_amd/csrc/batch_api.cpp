@@ -41,7 +41,7 @@ struct Dom {
   float* m[2] = {nullptr, nullptr};
   uint8_t* flags = nullptr;
   uint8_t* stage = nullptr;
-  uint32_t* nbm = nullptr;  // populations domains
+  uint32_t* nbm = nullptr;  // solid-neighbour masks
   float* inlet = nullptr;
   std::vector<uint8_t> hflags;
   bool outletTileStart = false;  // population domain: an outlet starts a pair-tile row
@@ -261,7 +261,7 @@ int fs_batch_create(const fs_domain_desc* descs, int ndomains, fs_batch** out) {
       offs.push_back(carve(P));
       offs.push_back(carve(P));
       offs.push_back(carve(sizeof(float) * size_t(s.ny) * s.nz));
-      offs.push_back(s.method == fs::kMethodHOME ? 0 : carve(sizeof(uint32_t) * P));
+      offs.push_back(carve(sizeof(uint32_t) * P));
       offs.push_back(s.method == fs::kMethodHOME ? 0 : carve(sizeof(float) * 19 * P));
     }
     if (hipStreamCreate(&b->stream) != hipSuccess || hipStreamCreate(&b->side) != hipSuccess ||
@@ -284,10 +284,8 @@ int fs_batch_create(const fs_domain_desc* descs, int ndomains, fs_batch** out) {
       D.flags = reinterpret_cast<uint8_t*>(b->arena + o[2]);
       D.stage = reinterpret_cast<uint8_t*>(b->arena + o[3]);
       D.inlet = reinterpret_cast<float*>(b->arena + o[4]);
-      if (D.desc.method != fs::kMethodHOME) {
-        D.nbm = reinterpret_cast<uint32_t*>(b->arena + o[5]);
-        D.F1 = reinterpret_cast<float*>(b->arena + o[6]);
-      }
+      D.nbm = reinterpret_cast<uint32_t*>(b->arena + o[5]);
+      if (D.desc.method != fs::kMethodHOME) D.F1 = reinterpret_cast<float*>(b->arena + o[6]);
       const float om = 1.0f / D.desc.tau;
       const fs::Phys ph = {om, om, om, D.desc.fx, D.desc.fy, D.desc.fz, FS_BOUNCE_HALFWAY};
       if (D.desc.method == fs::kMethodHOME) {
@@ -297,6 +295,7 @@ int fs_batch_create(const fs_domain_desc* descs, int ndomains, fs_batch** out) {
         h.m0 = D.m[0];
         h.m1 = D.m[1];
         h.flags = D.flags;
+        h.nbm = D.nbm;
         h.inlet = D.inlet;
         h.tile0 = uint32_t(homeTile.size());
         int ty = 0;

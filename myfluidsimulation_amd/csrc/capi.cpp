@@ -194,7 +194,7 @@ static int solver_pop_variant(const fs_solver* s) {
   return blocked ? fs::kPopAA : v;
 }
 
-static int solver_home_variant(const fs_solver* s) {
+int solver_home_variant(const fs_solver* s) {
   if (s->outlet_tile_start) return fs::kHomePlain;
   return s->home_variant >= 0 ? s->home_variant : fs::home_default_variant(s->g);
 }
@@ -206,7 +206,7 @@ int enqueue_steps(fs_solver* s, int nsteps) {
   if (s->method == fs::kMethodHOME) {
     const int variant = solver_home_variant(s);
     for (int i = 0; i < nsteps; ++i) {
-      fs::HomeLaunch a{s->mbuf[s->cur], s->mbuf[1 - s->cur], s->flags, s->inlet};
+      fs::HomeLaunch a{s->mbuf[s->cur], s->mbuf[1 - s->cur], s->flags, s->nbm, s->inlet};
       FS_HIP(fs::launch_home_step(s->g, s->ph, a, variant, s->stream));
       s->cur = 1 - s->cur;
     }
@@ -305,8 +305,8 @@ int fs_solver_create(int method, int nx, int ny, int nz, float tau, float fx, fl
       e = hipMalloc((void**)&s->F, sizeof(float) * 19 * P);
       if (e == hipSuccess) e = hipMalloc((void**)&s->F2, sizeof(float) * 19 * P);
       if (e == hipSuccess) e = hipMalloc((void**)&s->macro, sizeof(float) * 4 * P);
-      if (e == hipSuccess) e = hipMalloc((void**)&s->nbm, sizeof(uint32_t) * P);
     }
+    if (e == hipSuccess) e = hipMalloc((void**)&s->nbm, sizeof(uint32_t) * P);
     if (e == hipSuccess) e = hipMalloc((void**)&s->flags, P);
     if (e == hipSuccess) e = hipMalloc((void**)&s->stage, P);
     if (e == hipSuccess) e = hipMalloc((void**)&s->inlet, sizeof(float) * size_t(ny) * nz);

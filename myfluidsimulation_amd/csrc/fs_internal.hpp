@@ -153,6 +153,7 @@ struct HomeLaunch {
   const float* m;
   float* mn;
   const uint8_t* flags;
+  const uint32_t* nbm;  // bit q: the periodic neighbour c + e_q is solid
   const float* inlet;
 };
 // HOME kernel variants: the plain per-cell gather and three tile shapes
@@ -196,6 +197,7 @@ struct BatchHomeDomain {
   float* m0;
   float* m1;
   const uint8_t* flags;
+  const uint32_t* nbm;
   const float* inlet;
   uint32_t tile0;
   int tilesX, chunks;

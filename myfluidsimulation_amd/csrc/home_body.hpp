@@ -182,6 +182,11 @@ __device__ __forceinline__ void home_tile(const Geo& g, const Phys& ph, const Ho
     // is issued here, so the reconstruction and the barrier hide its latency
     float uinFin = 0.0f;
     if (active && (flPrev & F_INLET)) uinFin = a.inlet[size_t((p - 1 + g.nz) % g.nz) * g.ny + y];
+    // likewise a near-wall cell's solid-neighbour mask (one word instead of
+    // 18 neighbour flag bytes read where they are needed)
+    uint32_t nbmFin = 0u;
+    if (active && (flPrev & F_NEAR))
+      nbmFin = a.nbm[(uint32_t((p - 1 + g.nz) % g.nz) * g.ny + y) * g.nx + x];
     const Mom hm = nxtH;
     if (p < z0 + ZCe) load_plane(p + 1);
     if (active) {
@@ -244,11 +249,11 @@ __device__ __forceinline__ void home_tile(const Geo& g, const Phys& ph, const Ho
           if constexpr (CZ[q] < 0) f[q] = Cd[kSlot[q]];
         });
         if (anyNear) {
-          const Nbr on = nbr_offsets(x, y, ow, g);
+          // c - e_q is the neighbour along e_opp(q)
           for_q<1, 19>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            const uint32_t s = co + nb_off<-CX[q], -CY[q], -CZ[q]>(on);
-            if (a.flags[s] & F_SOLID) f[q] = home_rec<BB == 0 ? OPP[q] : OPP_REF[q]>(mPrev, omm);
+            if ((nbmFin >> OPP[q]) & 1u)
+              f[q] = home_rec<BB == 0 ? OPP[q] : OPP_REF[q]>(mPrev, omm);
           });
         }
         float r = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;

@@ -36,7 +36,7 @@ struct fs_solver {
   int home_variant = -1;  // HOME kernel variant, -1 = default for the grid
   // flags
   uint8_t* flags = nullptr;  // derived flag byte used by the kernels
-  uint32_t* nbm = nullptr;   // solid-neighbour masks (populations methods)
+  uint32_t* nbm = nullptr;   // solid-neighbour masks (bit q: c + e_q is solid)
   uint8_t* stage = nullptr;  // public flag bytes uploaded from the host
   std::vector<uint8_t> hflags;
   // an outlet cell starts a 64-wide (32-wide) row, x % 64 == 0 (x % 32 == 0):
@@ -52,6 +52,9 @@ struct fs_solver {
 
 // Kernel arguments of the population step on the solver's own buffers.
 fs::PopLaunch pop_args(fs_solver* s);
+// The HOME kernel variant the solver runs (its choice or the grid default;
+// the plain gather while an outlet starts a 64-wide tile row).
+int solver_home_variant(const fs_solver* s);
 // Re-initialises the solver's state from macro fields (device pointers,
 // nullable) without synchronising the stream.
 int solver_reinit_async(fs_solver* s, const float* rho, const float* ux, const float* uy,
