@@ -74,10 +74,14 @@ int pop_default_variant(const Geo& g) {
   // A workgroup walks its whole z-chunk, so a small grid gives too few of
   // them to fill 256 CUs: below 512 tiles of 64x4x16 (measured crossover,
   // about 128^3) the one-step kernel, with one workgroup per 256 cells, is
-  // faster.  Above it 64x4x16 tiles measured fastest on every workload
-  // (64x8 tiles re-read fewer halo rows but fit one workgroup per CU).
-  const long tiles4 = long((g.nx + 63) / 64) * ((g.ny + 3) / 4) * ((g.nz + 15) / 16);
-  return tiles4 < 512 ? kPopAA : kPairT4Z16;
+  // faster.  64x8 tiles with 32-plane chunks re-read the fewest halo rows and
+  // planes and win once they still give >= 4 workgroups per CU (256^3 and
+  // up); otherwise 64x4x16 (the 384x128x128 channel gives 384 of the former).
+  const long cols = (g.nx + 63) / 64;
+  const long tiles4 = cols * ((g.ny + 3) / 4) * ((g.nz + 15) / 16);
+  if (tiles4 < 512) return kPopAA;
+  const long tiles8z32 = cols * ((g.ny + 7) / 8) * ((g.nz + 31) / 32);
+  return tiles8z32 >= 1024 ? kPairT8Z32 : kPairT4Z16;
 }
 
 hipError_t launch_pop_pair(const Geo& g, const Phys& ph, const PairLaunch& a, bool hybrid,
