@@ -393,3 +393,35 @@ def test_outlet_at_narrow_tile_row_start(fs, oracle, method):
         res = run_pair(fs, oracle, method, (nx, ny, nz), 0.6, 8, flags=flags, fields=fields,
                        inlet=inlet, variant=variant)
         assert_exact(res, f"outlet at x=96, variant {variant}")
+
+
+@pytest.mark.parametrize("where", ["feeder", "outlet"])
+def test_hybrid_band_next_to_outlet(fs, oracle, where):
+    """An outlet collides as its upstream cell, band bit included: the band
+    marked on the feeder column only, or on the outlet column only, gives the
+    oracle's bits with the one-step kernel and both two-step tile widths."""
+    nx, ny, nz = 100, 8, 6
+    flags = wl.channel_flags(nx, ny, nz).reshape(nz, ny, nx)
+    col = nx - 2 if where == "feeder" else nx - 1
+    flags[:, :, col] |= np.where((flags[:, :, col] & 1) == 0, 2, 0).astype(np.uint8)
+    flags = flags.reshape(-1)
+    inlet = np.full(ny * nz, 0.04, np.float32)
+    fields = wl.uniform_flow(nx * ny * nz, 0.04)
+    o = oracle.Oracle(2, nx, ny, nz, 0.6, (1e-6, 0.0, 0.0))
+    o.set_flags(flags)
+    o.set_inlet(inlet)
+    o.set_omegas(1.0 / 0.6, 1.25, 1.5)
+    o.reinit(*fields)
+    o.step(6)
+    for variant in (0, 1, 5):
+        with fs.Solver(2, nx, ny, nz, 0.6, (1e-6, 0.0, 0.0)) as s:
+            s.set_flags(flags)
+            s.set_inlet_profile(inlet)
+            s.set_omegas(1.0 / 0.6, 1.25, 1.5)
+            s.set_variant(variant)
+            s.reinit_from_macro(*fields)
+            s.step(6)
+            np.testing.assert_array_equal(s.state(), o.state(), err_msg=f"variant {variant}")
+            for g, r in zip(s.macros(), o.macros()):
+                np.testing.assert_array_equal(g, r, err_msg=f"variant {variant}")
+    o.close()
