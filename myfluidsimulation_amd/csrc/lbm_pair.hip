@@ -6,8 +6,8 @@
 // Step t's post-collision populations live only in LDS.
 //
 // A workgroup owns a TX x TY column tile (TX = 64 or 32) and a chunk of ZC
-// planes and walks
-// the planes z0-1 .. z0+ZC:
+// planes and walks the planes z0-1 .. z0+ZC (the tile body is pair_tile in
+// pair_body.hpp, shared with the batched kernel):
 //   1. every tile cell, and the ring of cells around the tile, reads its f^t
 //      (own slots, reference order) and collides (step t); the tile cells put
 //      all 19 post-collision values into LDS, the ring cells only the five
@@ -25,11 +25,13 @@
 // launches of those equal one launch of this bit for bit.
 //
 // Boundaries follow the oracle's rule for the channel extension: an inlet
-// cell emits feq(1, (u_in, 0, 0)) whatever streams into it; an outlet cell
-// collides as its upstream neighbour x-1 (in step 1 it reads that cell's f^t,
-// in step 2 it takes the neighbour lane's result), so an outlet must not
-// start a tile row (the caller checks).  Solid cells are never
-// written: their frozen slots must already be equal in both buffers.
+// cell emits feq(1, (u_in, 0, 0)) whatever streams into it (it runs the fluid
+// code on zero inputs with PopCollision::set_inlet, which gives exactly that
+// feq); an outlet cell collides as its upstream neighbour x-1 (in step 1 it
+// reads that cell's f^t, in step 2 it takes the neighbour lane's gathered
+// f^{t+1}), so an outlet must not start a tile row (the caller checks).
+// Solid cells are never written: their frozen slots must already be equal in
+// both buffers.
 #include "pair_body.hpp"
 
 namespace fs {
