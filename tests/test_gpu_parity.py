@@ -425,3 +425,57 @@ def test_hybrid_band_next_to_outlet(fs, oracle, where):
             for g, r in zip(s.macros(), o.macros()):
                 np.testing.assert_array_equal(g, r, err_msg=f"variant {variant}")
     o.close()
+
+
+def random_pop_case(seed):
+    """A small random domain: solid blobs, an optional inlet/outlet channel,
+    Hybrid band cells, forcing and perturbed fields (seeded)."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = int(rng.integers(6, 90)), int(rng.integers(3, 20)), int(rng.integers(1, 18))
+    solid = (rng.random((nz, ny, nx)) < 0.08).astype(np.uint8)
+    flags = solid.copy()
+    if rng.random() < 0.6 and nx >= 4:
+        flags[:, :, 0] = np.where(solid[:, :, 0] == 0, 4, 1)
+        ok = (solid[:, :, nx - 1] == 0) & (flags[:, :, nx - 2] == 0)
+        flags[:, :, nx - 1] = np.where(ok, 8, flags[:, :, nx - 1])
+    band = (rng.random((nz, ny, nx)) < 0.3) & ((flags & 1) == 0)
+    flags = (flags | (band * 2)).astype(np.uint8).reshape(-1)
+    n = nx * ny * nz
+    fields = (np.float32(1) + np.float32(0.01) * rng.standard_normal(n, dtype=np.float32),
+              np.float32(0.03) * rng.standard_normal(n, dtype=np.float32),
+              np.float32(0.03) * rng.standard_normal(n, dtype=np.float32),
+              np.float32(0.03) * rng.standard_normal(n, dtype=np.float32))
+    inlet = (np.float32(0.02) + np.float32(0.02) * rng.random(ny * nz, dtype=np.float32))
+    method = int(rng.choice([0, 2]))
+    variant = seed % 6
+    chunks = [int(k) for k in rng.integers(1, 6, size=3)]
+    force = (float(rng.normal(0, 1e-5)), float(rng.normal(0, 1e-5)), 0.0)
+    return (nx, ny, nz), flags, fields, inlet, method, variant, chunks, force
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_pop_random_geometries(fs, oracle, seed):
+    """Seeded random small domains through every BGK / Hybrid kernel variant
+    and step chunking: the oracle's bits."""
+    dims, flags, fields, inlet, method, variant, chunks, force = random_pop_case(seed)
+    o = oracle.Oracle(method, *dims, 0.57, force)
+    o.set_flags(flags)
+    o.set_inlet(inlet)
+    if method == 2:
+        o.set_omegas(1.0 / 0.57, 1.3, 1.6)
+    o.reinit(*fields)
+    with fs.Solver(method, *dims, 0.57, force) as s:
+        s.set_flags(flags)
+        s.set_inlet_profile(inlet)
+        if method == 2:
+            s.set_omegas(1.0 / 0.57, 1.3, 1.6)
+        s.set_variant(variant)
+        s.reinit_from_macro(*fields)
+        for k in chunks:
+            s.step(k)
+            o.step(k)
+            what = f"seed {seed} dims {dims} method {method} variant {variant} chunk {k}"
+            np.testing.assert_array_equal(s.state(), o.state(), err_msg=what)
+            for g, r in zip(s.macros(), o.macros()):
+                np.testing.assert_array_equal(g, r, err_msg=what)
+    o.close()
