@@ -134,3 +134,34 @@ def test_batch_outlet_at_pair_tile_start(fs):
             np.testing.assert_array_equal(b.state(i), st, err_msg=f"dom {i} state")
             for k in range(4):
                 np.testing.assert_array_equal(b.macro(i, k), mac[k], err_msg=f"dom {i} macro {k}")
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_batch_random_domains(fs, seed):
+    """Seeded random small domains of every method in one batch, stepped in
+    random chunks: every domain equals its single-domain solver."""
+    import sys, os
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from test_gpu_parity import random_pop_case
+
+    rng = np.random.default_rng(500 + seed)
+    doms = []
+    for i in range(4):
+        dims, flags, fields, inlet, method, _, _, force = random_pop_case(1000 + 10 * seed + i)
+        method = int(rng.integers(0, 3))
+        nx = dims[0]
+        if method == 1:
+            flags = flags & np.uint8(0xFD)
+            if (nx - 1) % 32 == 0:  # a HOME outlet may not start a 32-wide tile
+                flags = np.where((flags & 12) != 0, 0, flags).astype(np.uint8)
+        doms.append(dict(method=method, dims=dims, tau=0.6 + 0.1 * i, flags=flags,
+                         fields=fields, inlet=inlet, force=force))
+    chunks = [int(k) for k in rng.integers(1, 8, size=3)]
+    with make_batch(fs, doms) as b:
+        for k in chunks:
+            b.step(k)
+        for i, d in enumerate(doms):
+            mac, st = single(fs, d, sum(chunks))
+            np.testing.assert_array_equal(b.state(i), st, err_msg=f"seed {seed} dom {i}")
+            for q in range(4):
+                np.testing.assert_array_equal(b.macro(i, q), mac[q], err_msg=f"seed {seed} dom {i}")

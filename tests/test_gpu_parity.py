@@ -479,3 +479,33 @@ def test_pop_random_geometries(fs, oracle, seed):
             for g, r in zip(s.macros(), o.macros()):
                 np.testing.assert_array_equal(g, r, err_msg=what)
     o.close()
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_home_random_geometries(fs, oracle, seed):
+    """Seeded random small domains through every HOME kernel variant and both
+    bounce-back rules: the oracle's bits."""
+    dims, flags, fields, inlet, _, _, chunks, force = random_pop_case(100 + seed)
+    flags = flags & np.uint8(0xFD)  # HOME has no band
+    variant, rule = seed % 4, seed % 2
+    o = oracle.Oracle(1, *dims, 0.61, force)
+    o.set_flags(flags)
+    o.set_inlet(inlet)
+    o.set_bounce(rule)
+    o.reinit(*fields)
+    with fs.Solver(1, *dims, 0.61, force) as s:
+        s.set_flags(flags)
+        s.set_inlet_profile(inlet)
+        s.set_bounce_rule(rule)
+        nx = dims[0]
+        if variant == 0 or (nx - 1) % 64 != 0:
+            s.set_variant(variant)
+        s.reinit_from_macro(*fields)
+        for k in chunks:
+            s.step(k)
+            o.step(k)
+            what = f"seed {seed} dims {dims} variant {variant} rule {rule} chunk {k}"
+            np.testing.assert_array_equal(s.state(), o.state(), err_msg=what)
+            for g, r in zip(s.macros(), o.macros()):
+                np.testing.assert_array_equal(g, r, err_msg=what)
+    o.close()
