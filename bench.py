@@ -16,6 +16,8 @@ Other workloads (--workload), the large single-domain roofline runs:
   batch64     the 64-domain batched sweep against the same runs one by one
   adjoint128  differentiable 128^3 Taylor-Green run (T = 2000, sqrt(T)
               checkpoints): forward + backward MLUPS against plain steps
+  voxel       the geometry path: STL parse, voxelization (and AMR 2), band,
+              wall ms per C-ABI call against the single-threaded oracle
 The default channel line also carries a shorter run of the batched sweep.
 
 Timing: HIP events recorded on the solver's stream around the K steps
@@ -304,6 +306,76 @@ def adjoint_bench(method, n, steps, segment=0):
             "grad_finite": bool(np.isfinite(g).all()), "grad_norm": float(np.linalg.norm(g))}
 
 
+VOXEL_CASES = [  # (label, nx, ny, nz, centre, radius, refine)
+    ("channel384", 384, 128, 128, (96, 64, 64), 20.0, 1),
+    ("channel384_amr2", 384, 128, 128, (96, 64, 64), 20.0, 2),
+    ("channel768", 768, 256, 256, (192, 128, 128), 40.0, 1),
+]
+
+
+def voxel_bench(reps, cpu=True):
+    """The geometry path of the channel workloads: binary STL parse
+    (load_stl), voxelization (voxelize_mesh_to_mask, optionally AMR r=2) and
+    the Hybrid band (d0 = 2), each a whole C-ABI call (host mesh in, host mask
+    out, PCIe included): median of `reps` calls after one warm-up.  The CPU
+    leg times the single-threaded oracle restatement (as the reference runs
+    these on one host thread) on the same inputs once, and compares masks."""
+    from myfluidsimulation_amd import solver as S
+    from myfluidsimulation_amd import workloads as wl
+
+    def median_ms(fn):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return round(1e3 * float(np.median(ts)), 3)
+
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle  # the checker, used here only as the timed CPU baseline
+    cases = {}
+    with tempfile.TemporaryDirectory() as td:
+        for label, nx, ny, nz, center, radius, refine in VOXEL_CASES:
+            path = os.path.join(td, f"{label}.stl")
+            scale, trans = wl.obstacle_stl(path, nx, ny, nz, center, radius, SEED)
+            tris, bmin, bmax = S.load_stl(path)
+            vox = lambda: S.voxelize(tris, bmin, bmax, nx, ny, nz, scale, trans, refine)  # noqa: E731
+            solid = vox()
+            r = {"cells": nx * ny * nz, "triangles": int(tris.shape[0]), "refine": refine,
+                 "solid_cells": int(solid.sum()),
+                 "load_stl_ms": median_ms(lambda: S.load_stl(path)),
+                 "voxelize_ms": median_ms(vox),
+                 "band2_ms": median_ms(lambda: S.band(solid, nx, ny, nz, 2))}
+            if cpu:
+                t0 = time.perf_counter()
+                ref = pyoracle.voxelize(tris, bmin, bmax, nx, ny, nz, scale, trans, refine)
+                r["cpu_voxelize_ms"] = round(1e3 * (time.perf_counter() - t0), 1)
+                t0 = time.perf_counter()
+                ref_band = pyoracle.band(ref, nx, ny, nz, 2)
+                r["cpu_band2_ms"] = round(1e3 * (time.perf_counter() - t0), 1)
+                r["identical"] = bool(np.array_equal(ref, solid)
+                                      and np.array_equal(ref_band, S.band(solid, nx, ny, nz, 2)))
+                r["voxelize_speedup"] = round(r["cpu_voxelize_ms"] / r["voxelize_ms"], 1)
+            cases[label] = r
+    head = cases["channel768"]
+    line = {
+        "metric": "voxelize wall ms per call (STL mesh -> solid mask, C ABI, host in/out)",
+        "value": head["voxelize_ms"], "unit": "ms", "n_gpus": 1, "steps": reps, "warmup": 1,
+        "ms_per_step": head["voxelize_ms"], "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded)",
+        "config": {"workload": "jittered icosphere obstacles (1280 faces) of the 384x128x128 "
+                               "and 768x256x256 channels; value = 768x256x256"},
+        "cases": cases,
+    }
+    if cpu:
+        line["cpu_baseline"] = {"value": head["cpu_voxelize_ms"], "unit": "ms", "cores": 1,
+                                "kind": "port",
+                                "sample": "one voxelization of each case by the oracle"}
+    return line
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -311,7 +383,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", default="channel",
                     choices=["channel", "tgv256", "tgv512", "channel768", "batch64",
-                             "adjoint128"])
+                             "adjoint128", "voxel"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the 64-domain batched sweep summary of the channel workload")
@@ -340,6 +412,9 @@ def main():
             "data": "synthetic (seeded)",
             "config": {"workload": "Taylor-Green 128^3 inverse problem, dJ/du0, value = BGK"},
             "methods": {r["method"]: r for r in runs}}))
+        return
+    if args.workload == "voxel":
+        print(json.dumps(voxel_bench(args.steps or 5, cpu=not args.no_cpu)))
         return
     if args.workload == "batch64":
         print(json.dumps(batch_bench(args.steps or 2000, 20 if args.warmup is None

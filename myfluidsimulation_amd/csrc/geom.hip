@@ -113,25 +113,127 @@ __global__ void __launch_bounds__(256) k_seed(const uint8_t* __restrict__ surf, 
   vis[i] = (face && !surf[i]) ? 1 : 0;
 }
 
-// one sweep along axis `axis` per line, both directions
-__global__ void __launch_bounds__(256) k_sweep(const uint8_t* __restrict__ surf,
-                                               uint8_t* __restrict__ vis, int nx, int ny, int nz,
-                                               int axis, int* __restrict__ changed) {
+// Sets the round's "something changed" flag.  In a round that fills the
+// exterior nearly every wavefront reaches new cells, and one atomic each on
+// the same word serialises them (measured: the first x sweep of a 50 M cell
+// grid took 0.74 ms, the round after it 0.07 ms); the flag only ever goes
+// 0 -> 1, so once a lane sees it set there is nothing to do.
+__device__ __forceinline__ void mark_changed(int* changed) {
+  if (__hip_atomic_load(changed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+    atomicOr(changed, 1);
+}
+
+// One sweep per line, forward then backward: a cell becomes reached when the
+// cell before it (in the sweep direction) is reached and it is not surface.
+// After both passes a line's reached cells are exactly the open runs (maximal
+// runs of non-surface cells) that held a reached cell.
+
+// Lines along x, one wavefront per line, 64 cells per step as bit masks
+// (bit l = lane l = cell x0 + l).  Within a word, the cells reached from the
+// seeds s (a subset of the open cells o) in the direction of higher bits are
+// s | (o & ((o + s) ^ o)): the carry of o + s runs from each seed through
+// the rest of its open run and stops at the first closed bit.  A reached bit
+// 63 carries into bit 0 of the next word; the backward pass is the same on
+// bit-reversed words.  The cells of kSweepXBatch words are loaded before the
+// first of them is processed, so their loads overlap.  Each lane re-reads
+// only its own cells, which it wrote itself in the forward pass.
+constexpr int kSweepXBatch = 4;
+
+// one word of the x sweep: cell bytes (vl, sl) of this lane, `in` false for
+// lanes past the line end; REV: the backward direction
+template <bool REV>
+__device__ __forceinline__ void sweep_word(uint8_t vl, uint8_t sl, bool in, int lane,
+                                           uint8_t* __restrict__ cell, uint64_t& carry,
+                                           bool& any) {
+  const uint8_t v = in ? vl : uint8_t(0);
+  const uint8_t s = in ? sl : uint8_t(1);  // lanes past the line end are closed
+  uint64_t open = __ballot(s == 0), seen = __ballot(v != 0);
+  if (REV) {
+    open = __builtin_bitreverse64(open);
+    seen = __builtin_bitreverse64(seen);
+  }
+  const uint64_t seeds = (seen | carry) & open;
+  uint64_t reach = seeds | (open & ((open + seeds) ^ open));
+  carry = reach >> 63;
+  if (REV) reach = __builtin_bitreverse64(reach);
+  if (in && !v && ((reach >> lane) & 1ull)) {
+    *cell = 1;
+    any = true;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_sweep_x(const uint8_t* __restrict__ surf,
+                                                 uint8_t* __restrict__ vis, int nx, int nlines,
+                                                 int* __restrict__ changed) {
+  const int lane = int(threadIdx.x) & 63;
+  const long line = long(blockIdx.x) * 4 + (threadIdx.x >> 6);
+  // wavefronts past the last line run the same steps with every lane
+  // inactive (their ballots are 0; they read cell 0 and write nothing)
+  const bool lineOk = line < nlines;
+  const size_t base = lineOk ? size_t(line) * size_t(nx) : 0;
+  const int chunks = (nx + 63) / 64;
+  bool any = false;
+  uint64_t carry = 0;
+  for (int c0 = 0; c0 < chunks; c0 += kSweepXBatch) {
+    uint8_t vl[kSweepXBatch], sl[kSweepXBatch];
+#pragma unroll
+    for (int j = 0; j < kSweepXBatch; ++j) {
+      const int x = (c0 + j) * 64 + lane;
+      const size_t i = base + size_t(lineOk && x < nx ? x : 0);  // always a valid cell
+      vl[j] = vis[i];
+      sl[j] = surf[i];
+    }
+#pragma unroll
+    for (int j = 0; j < kSweepXBatch; ++j) {
+      if (c0 + j < chunks) {  // uniform across the wavefront
+        const int x = (c0 + j) * 64 + lane;
+        const bool in = lineOk && x < nx;
+        sweep_word<false>(vl[j], sl[j], in, lane, vis + base + (in ? x : 0), carry, any);
+      }
+    }
+  }
+  carry = 0;
+  for (int c1 = chunks - 1; c1 >= 0; c1 -= kSweepXBatch) {
+    uint8_t vl[kSweepXBatch], sl[kSweepXBatch];
+#pragma unroll
+    for (int j = 0; j < kSweepXBatch; ++j) {
+      const int x = (c1 - j) * 64 + lane;
+      const size_t i = base + size_t(lineOk && c1 - j >= 0 && x < nx ? x : 0);
+      vl[j] = vis[i];
+      sl[j] = surf[i];
+    }
+#pragma unroll
+    for (int j = 0; j < kSweepXBatch; ++j) {
+      if (c1 - j >= 0) {
+        const int x = (c1 - j) * 64 + lane;
+        const bool in = lineOk && x < nx;
+        sweep_word<true>(vl[j], sl[j], in, lane, vis + base + (in ? x : 0), carry, any);
+      }
+    }
+  }
+  if (any) mark_changed(changed);
+}
+
+// Lines along y (axis 1, indexed by (x,z)) or z (axis 2, indexed by (x,y)),
+// one thread per line: neighbouring lanes hold neighbouring x, so every
+// access is coalesced across the wavefront.  Cells are loaded kSweepBatch at
+// a time before any of them is processed, so the loads of a batch overlap
+// (a line's cells are its thread's alone: no other thread writes them).
+constexpr int kSweepBatch = 8;
+
+__global__ void __launch_bounds__(256) k_sweep_yz(const uint8_t* __restrict__ surf,
+                                                  uint8_t* __restrict__ vis, int nx, int ny,
+                                                  int nz, int axis, int* __restrict__ changed) {
   const size_t lid = size_t(blockIdx.x) * 256 + threadIdx.x;
   size_t base, stride;
   int len;
-  if (axis == 0) {  // lines along x, indexed by (y,z)
-    if (lid >= size_t(ny) * nz) return;
-    base = lid * nx;
-    stride = 1;
-    len = nx;
-  } else if (axis == 1) {  // along y, indexed by (x,z)
+  if (axis == 1) {
     if (lid >= size_t(nx) * nz) return;
     const size_t x = lid % nx, z = lid / nx;
     base = z * size_t(nx) * ny + x;
     stride = nx;
     len = ny;
-  } else {  // along z, indexed by (x,y)
+  } else {
     if (lid >= size_t(nx) * ny) return;
     base = lid;
     stride = size_t(nx) * ny;
@@ -139,28 +241,52 @@ __global__ void __launch_bounds__(256) k_sweep(const uint8_t* __restrict__ surf,
   }
   int any = 0;
   uint8_t prev = 0;
-  for (int k = 0; k < len; ++k) {
-    const size_t i = base + size_t(k) * stride;
-    uint8_t v = vis[i];
-    if (!v && prev && !surf[i]) {
-      v = 1;
-      vis[i] = 1;
-      any = 1;
+  for (int k0 = 0; k0 < len; k0 += kSweepBatch) {
+    uint8_t v[kSweepBatch], s[kSweepBatch];
+#pragma unroll
+    for (int j = 0; j < kSweepBatch; ++j) {
+      const bool in = k0 + j < len;
+      const size_t i = base + size_t(in ? k0 + j : 0) * stride;  // always a cell of the line
+      const uint8_t vl = vis[i], sl = surf[i];
+      v[j] = in ? vl : uint8_t(0);
+      s[j] = in ? sl : uint8_t(1);
     }
-    prev = v;
+#pragma unroll
+    for (int j = 0; j < kSweepBatch; ++j) {
+      if (k0 + j < len) {
+        if (!v[j] && prev && !s[j]) {
+          v[j] = 1;
+          vis[base + size_t(k0 + j) * stride] = 1;
+          any = 1;
+        }
+        prev = v[j];
+      }
+    }
   }
   prev = 0;
-  for (int k = len - 1; k >= 0; --k) {
-    const size_t i = base + size_t(k) * stride;
-    uint8_t v = vis[i];
-    if (!v && prev && !surf[i]) {
-      v = 1;
-      vis[i] = 1;
-      any = 1;
+  for (int k1 = len - 1; k1 >= 0; k1 -= kSweepBatch) {
+    uint8_t v[kSweepBatch], s[kSweepBatch];
+#pragma unroll
+    for (int j = 0; j < kSweepBatch; ++j) {
+      const bool in = k1 - j >= 0;
+      const size_t i = base + size_t(in ? k1 - j : 0) * stride;
+      const uint8_t vl = vis[i], sl = surf[i];
+      v[j] = in ? vl : uint8_t(0);
+      s[j] = in ? sl : uint8_t(1);
     }
-    prev = v;
+#pragma unroll
+    for (int j = 0; j < kSweepBatch; ++j) {
+      if (k1 - j >= 0) {
+        if (!v[j] && prev && !s[j]) {
+          v[j] = 1;
+          vis[base + size_t(k1 - j) * stride] = 1;
+          any = 1;
+        }
+        prev = v[j];
+      }
+    }
   }
-  if (any) atomicOr(changed, 1);
+  if (any) mark_changed(changed);
 }
 
 __global__ void __launch_bounds__(256) k_finalize(uint8_t* __restrict__ surf_io,
@@ -251,9 +377,11 @@ hipError_t voxelize_device(const TriF* d_tris, int ntri, int nx, int ny, int nz,
   // cells, so the loop ends
   for (size_t round = 0; round <= n; ++round) {
     if ((e = hipMemsetAsync(changed, 0, sizeof(int), st)) != hipSuccess) break;
-    for (int ax = 0; ax < 3; ++ax)
-      hipLaunchKernelGGL(k_sweep, grid1d(lines[ax]), dim3(256), 0, st, d_out, vis, nx, ny, nz, ax,
-                         changed);
+    hipLaunchKernelGGL(k_sweep_x, dim3(unsigned((lines[0] + 3) / 4)), dim3(256), 0, st, d_out,
+                       vis, nx, int(lines[0]), changed);
+    for (int ax = 1; ax < 3; ++ax)
+      hipLaunchKernelGGL(k_sweep_yz, grid1d(lines[ax]), dim3(256), 0, st, d_out, vis, nx, ny, nz,
+                         ax, changed);
     int h = 0;
     if ((e = hipMemcpyAsync(&h, changed, sizeof(int), hipMemcpyDeviceToHost, st)) != hipSuccess)
       break;

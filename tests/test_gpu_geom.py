@@ -32,6 +32,79 @@ def test_voxelize_icosphere_exact(fs, oracle, tmp_path, refine):
     assert np.array_equal(g, o), f"{int((g != o).sum())} voxels differ"
 
 
+@pytest.mark.parametrize("shape", [(1, 12, 9), (2, 5, 30), (63, 17, 4), (64, 9, 7), (65, 11, 6),
+                                   (129, 8, 5), (257, 4, 4), (520, 5, 4)])
+def test_voxelize_line_lengths_exact(fs, oracle, tmp_path, shape):
+    """x-line lengths around multiples of 64 cells and of 256 (the x sweep
+    handles a line 64 cells at a time, four words per batch)."""
+    nx, ny, nz = shape
+    for k in range(2):
+        center = (nx * (0.35 + 0.3 * k), ny * 0.5, nz * 0.45)
+        radius = max(1.5, 0.35 * min(nx, ny, nz)) * (0.8 + 0.4 * k)
+        path, scale, trans = _stl(str(tmp_path), nx, ny, nz, center, radius, 40 + k)
+        tris, bmin, bmax = fs.load_stl(path)
+        g = fs.voxelize(tris, bmin, bmax, nx, ny, nz, scale, trans)
+        o = oracle.voxelize(tris, bmin, bmax, nx, ny, nz, scale, trans)
+        assert np.array_equal(g, o), f"{int((g != o).sum())} voxels differ"
+
+
+def _box(lo, hi):
+    """12 triangles of an axis-aligned box."""
+    (x0, y0, z0), (x1, y1, z1) = lo, hi
+    v = np.array([[x0, y0, z0], [x1, y0, z0], [x1, y1, z0], [x0, y1, z0],
+                  [x0, y0, z1], [x1, y0, z1], [x1, y1, z1], [x0, y1, z1]], np.float32)
+    faces = [(0, 1, 2), (0, 2, 3), (4, 6, 5), (4, 7, 6), (0, 4, 5), (0, 5, 1),
+             (3, 2, 6), (3, 6, 7), (0, 3, 7), (0, 7, 4), (1, 5, 6), (1, 6, 2)]
+    return np.stack([v[list(f)].reshape(9) for f in faces])
+
+
+def _shell(c0, c1, t, hole_side):
+    """Hollow cube [c0, c1]^3 with walls of thickness t; hole_side +1 / -1
+    cuts a square opening (y, z in [0.4, 0.6]) into the +x / -x wall, 0 none."""
+    parts = []
+    for axis in range(3):
+        for side in (0, 1):
+            lo, hi = [c0] * 3, [c1] * 3
+            if side == 0:
+                hi[axis] = c0 + t
+            else:
+                lo[axis] = c1 - t
+            if axis == 0 and hole_side == (1 if side else -1):
+                a, b = 0.4, 0.6  # the wall minus the opening: four slabs
+                for ylo, yhi, zlo, zhi in ((c0, a, c0, c1), (b, c1, c0, c1), (a, b, c0, a),
+                                           (a, b, b, c1)):
+                    parts.append(_box((lo[0], ylo, zlo), (hi[0], yhi, zhi)))
+            else:
+                parts.append(_box(lo, hi))
+    return np.concatenate(parts)
+
+
+@pytest.mark.parametrize("shape", [(64, 64, 64), (130, 48, 40)])
+def test_voxelize_nested_shells_exact(fs, oracle, tmp_path, shape):
+    """Nested hollow cubes with openings on opposite sides around a closed
+    one: the exterior reaches the space between the outer shells only by
+    turning corners (several rounds of line sweeps), and the closed shell's
+    inside is an enclosed cavity that ends up solid."""
+    nx, ny, nz = shape
+    soup = np.concatenate([_shell(0.1, 0.9, 0.05, +1), _shell(0.25, 0.75, 0.05, -1),
+                           _shell(0.4, 0.6, 0.04, 0)])
+    path = os.path.join(str(tmp_path), "shells.stl")
+    wl.write_binary_stl(path, soup)
+    tris, bmin, bmax = fs.load_stl(path)
+    scale = float((bmax - bmin).max())
+    trans = tuple(float(v) for v in 0.5 * (bmin + bmax))
+    g = fs.voxelize(tris, bmin, bmax, nx, ny, nz, scale, trans)
+    o = oracle.voxelize(tris, bmin, bmax, nx, ny, nz, scale, trans)
+    assert np.array_equal(g, o), f"{int((g != o).sum())} voxels differ"
+    m = g.reshape(nz, ny, nx)
+    # between the outer two shells (reached through both openings) is fluid,
+    # the innermost cube's centre is solid
+    assert m[nz // 2, ny // 2, int(0.18 * nx)] == 0
+    assert m[nz // 2, int(0.18 * ny), nx // 2] == 0
+    assert m[nz // 2, ny // 2, int(0.32 * nx)] == 0
+    assert m[nz // 2, ny // 2, nx // 2] == 1
+
+
 def test_voxelize_reference_defaults_exact(fs, oracle, tmp_path):
     """Default VoxelParams path (scale 0.5 about the centre) on a non-cubic
     grid, plus thresholds 0.25 / 1.0 for the AMR downsample."""
