@@ -18,7 +18,8 @@ Other workloads (--workload), the large single-domain roofline runs:
               checkpoints): forward + backward MLUPS against plain steps
   voxel       the geometry path: STL parse, voxelization (and AMR 2), band,
               wall ms per C-ABI call against the single-threaded oracle
-The default channel line also carries a shorter run of the batched sweep.
+The default channel line also carries a shorter run of the batched sweep and
+the geometry path of its obstacle (`geometry`: voxelize / band ms per call).
 
 Timing: HIP events recorded on the solver's stream around the K steps
 (fs_solver_timed_steps).  HOME and the one-step BGK / Hybrid kernel launch
@@ -313,7 +314,7 @@ VOXEL_CASES = [  # (label, nx, ny, nz, centre, radius, refine)
 ]
 
 
-def voxel_bench(reps, cpu=True):
+def voxel_bench(reps, cpu=True, labels=None):
     """The geometry path of the channel workloads: binary STL parse
     (load_stl), voxelization (voxelize_mesh_to_mask, optionally AMR r=2) and
     the Hybrid band (d0 = 2), each a whole C-ABI call (host mesh in, host mask
@@ -338,6 +339,8 @@ def voxel_bench(reps, cpu=True):
     cases = {}
     with tempfile.TemporaryDirectory() as td:
         for label, nx, ny, nz, center, radius, refine in VOXEL_CASES:
+            if labels is not None and label not in labels:
+                continue
             path = os.path.join(td, f"{label}.stl")
             scale, trans = wl.obstacle_stl(path, nx, ny, nz, center, radius, SEED)
             tris, bmin, bmax = S.load_stl(path)
@@ -359,7 +362,7 @@ def voxel_bench(reps, cpu=True):
                                       and np.array_equal(ref_band, S.band(solid, nx, ny, nz, 2)))
                 r["voxelize_speedup"] = round(r["cpu_voxelize_ms"] / r["voxelize_ms"], 1)
             cases[label] = r
-    head = cases["channel768"]
+    head = cases["channel768" if "channel768" in cases else next(iter(cases))]
     line = {
         "metric": "voxelize wall ms per call (STL mesh -> solid mask, C ABI, host in/out)",
         "value": head["voxelize_ms"], "unit": "ms", "n_gpus": 1, "steps": reps, "warmup": 1,
@@ -494,6 +497,9 @@ def main():
                                              "sequential_async_MLUPS", "speedup_vs_sync_step1",
                                              "speedup_vs_async", "finite")}
         line["batched"]["workload"] = b["config"]["workload"]
+    if args.workload == "channel":
+        line["geometry"] = voxel_bench(3, cpu=not args.no_cpu,
+                                       labels=("channel384", "channel384_amr2"))["cases"]
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(case)
     print(json.dumps(line))
