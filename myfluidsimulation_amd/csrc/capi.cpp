@@ -737,9 +737,10 @@ int fs_band(const uint8_t* solid, int nx, int ny, int nz, int d0, uint8_t* band_
   uint8_t* d_scr = nullptr;
   hipError_t e = hipMalloc((void**)&d_solid, n);
   if (e == hipSuccess) e = hipMalloc((void**)&d_band, n);
-  if (e == hipSuccess) e = hipMalloc((void**)&d_scr, n * sizeof(int));
+  const int dd = d0 < 0 ? 0 : d0;
+  if (e == hipSuccess) e = hipMalloc((void**)&d_scr, fs::band_scratch_bytes(nx, ny, nz, dd));
   if (e == hipSuccess) e = hipMemcpy(d_solid, solid, n, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = fs::band_device(d_solid, nx, ny, nz, d0 < 0 ? 0 : d0, d_band, d_scr, nullptr);
+  if (e == hipSuccess) e = fs::band_device(d_solid, nx, ny, nz, dd, d_band, d_scr, nullptr);
   if (e == hipSuccess) e = hipMemcpy(band_out, d_band, n, hipMemcpyDeviceToHost);
   (void)hipFree(d_solid);
   (void)hipFree(d_band);

@@ -282,6 +282,9 @@ hipError_t voxelize_device(const TriF* d_tris, int ntri, int nx, int ny, int nz,
                            hipStream_t st);
 hipError_t downsample_device(const uint8_t* d_fine, int refine, int nx, int ny, int nz,
                              float threshold, uint8_t* d_out, hipStream_t st);
+// d_scratch: band_scratch_bytes(nx, ny, nz, d0) bytes (one byte per cell
+// while at most 254 distance rings are needed, an int per cell beyond)
+size_t band_scratch_bytes(int nx, int ny, int nz, int d0);
 hipError_t band_device(const uint8_t* d_solid, int nx, int ny, int nz, int d0, uint8_t* d_band,
                        uint8_t* d_scratch, hipStream_t st);
 

@@ -316,23 +316,35 @@ __global__ void __launch_bounds__(256) k_downsample(const uint8_t* __restrict__ 
   out[i] = ((cov > 0.0f) && (cov >= thr)) ? 1 : 0;
 }
 
-// band BFS: dist 0 on solid, -1 elsewhere
+// band BFS distances: D is uint8_t while the ring count fits below the
+// "unreached" value 255 (a quarter of the bytes), int otherwise
+template <class D>
+struct DistT {
+  static constexpr D kUnreached = D(-1);
+};
+
+// dist 0 on solid, unreached elsewhere
+template <class D>
 __global__ void __launch_bounds__(256) k_dist_init(const uint8_t* __restrict__ solid, size_t n,
-                                                   int* __restrict__ dist) {
+                                                   D* __restrict__ dist) {
   const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
   if (i >= n) return;
-  dist[i] = solid[i] ? 0 : -1;
+  dist[i] = solid[i] ? D(0) : DistT<D>::kUnreached;
 }
-// ring k: unreached cells with a 6-neighbour at k-1 (sim_runner.hpp:57-76)
-__global__ void __launch_bounds__(256) k_dist_ring(int* __restrict__ dist, int nx, int ny, int nz,
+// ring k: unreached cells with a 6-neighbour at k-1 (sim_runner.hpp:57-76);
+// cells are fewer than 2^31, so the index arithmetic is 32-bit
+template <class D>
+__global__ void __launch_bounds__(256) k_dist_ring(D* __restrict__ dist, int nx, int ny, int nz,
                                                    int k) {
-  const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
-  const size_t n = size_t(nx) * ny * nz;
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  const uint32_t n = uint32_t(nx) * uint32_t(ny) * uint32_t(nz);
   if (i >= n) return;
-  if (dist[i] != -1) return;
-  const int x = int(i % nx), y = int((i / nx) % ny), z = int(i / (size_t(nx) * ny));
-  const size_t sy = nx, sz = size_t(nx) * ny;
-  const int p = k - 1;
+  if (dist[i] != DistT<D>::kUnreached) return;
+  const uint32_t row = i / uint32_t(nx);
+  const int x = int(i - row * uint32_t(nx));
+  const int y = int(row % uint32_t(ny)), z = int(row / uint32_t(ny));
+  const uint32_t sy = uint32_t(nx), sz = uint32_t(nx) * uint32_t(ny);
+  const D p = D(k - 1);
   bool hit = false;
   if (x > 0) hit |= dist[i - 1] == p;
   if (x < nx - 1) hit |= dist[i + 1] == p;
@@ -340,15 +352,16 @@ __global__ void __launch_bounds__(256) k_dist_ring(int* __restrict__ dist, int n
   if (y < ny - 1) hit |= dist[i + sy] == p;
   if (z > 0) hit |= dist[i - sz] == p;
   if (z < nz - 1) hit |= dist[i + sz] == p;
-  if (hit) dist[i] = k;
+  if (hit) dist[i] = D(k);
 }
+template <class D>
 __global__ void __launch_bounds__(256) k_band_mask(const uint8_t* __restrict__ solid,
-                                                   const int* __restrict__ dist, size_t n, int d0,
+                                                   const D* __restrict__ dist, size_t n, int d0,
                                                    uint8_t* __restrict__ band) {
   const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
   if (i >= n) return;
-  const int d = dist[i];
-  band[i] = (!solid[i] && d >= 0 && d <= d0) ? 1 : 0;
+  const D d = dist[i];
+  band[i] = (!solid[i] && d != DistT<D>::kUnreached && int(d) <= d0) ? 1 : 0;
 }
 
 inline dim3 grid1d(size_t n) { return dim3(unsigned((n + 255) / 256)); }
@@ -408,18 +421,38 @@ hipError_t downsample_device(const uint8_t* d_fine, int refine, int nx, int ny, 
   return hipGetLastError();
 }
 
+namespace {
+
+// rings beyond d0 never enter the band, and no ring exceeds the grid's
+// Manhattan diameter
+int band_rings(int nx, int ny, int nz, int d0) { return d0 < nx + ny + nz ? d0 : nx + ny + nz; }
+
+template <class D>
+hipError_t band_run(const uint8_t* d_solid, int nx, int ny, int nz, int d0, uint8_t* d_band,
+                    D* dist, hipStream_t st) {
+  const size_t n = size_t(nx) * ny * nz;
+  hipLaunchKernelGGL(k_dist_init<D>, grid1d(n), dim3(256), 0, st, d_solid, n, dist);
+  const int rings = band_rings(nx, ny, nz, d0);
+  for (int k = 1; k <= rings; ++k)
+    hipLaunchKernelGGL(k_dist_ring<D>, grid1d(n), dim3(256), 0, st, dist, nx, ny, nz, k);
+  hipLaunchKernelGGL(k_band_mask<D>, grid1d(n), dim3(256), 0, st, d_solid, dist, n, d0, d_band);
+  return hipGetLastError();
+}
+
+bool band_bytes_dist(int nx, int ny, int nz, int d0) { return band_rings(nx, ny, nz, d0) < 255; }
+
+}  // namespace
+
+size_t band_scratch_bytes(int nx, int ny, int nz, int d0) {
+  const size_t n = size_t(nx) * ny * nz;
+  return band_bytes_dist(nx, ny, nz, d0) ? n : n * sizeof(int);
+}
+
 hipError_t band_device(const uint8_t* d_solid, int nx, int ny, int nz, int d0, uint8_t* d_band,
                        uint8_t* d_scratch, hipStream_t st) {
-  const size_t n = size_t(nx) * ny * nz;
-  int* dist = reinterpret_cast<int*>(d_scratch);
-  hipLaunchKernelGGL(k_dist_init, grid1d(n), dim3(256), 0, st, d_solid, n, dist);
-  // rings beyond d0 never enter the band, and no ring exceeds the grid's
-  // Manhattan diameter
-  const int rings = d0 < nx + ny + nz ? d0 : nx + ny + nz;
-  for (int k = 1; k <= rings; ++k)
-    hipLaunchKernelGGL(k_dist_ring, grid1d(n), dim3(256), 0, st, dist, nx, ny, nz, k);
-  hipLaunchKernelGGL(k_band_mask, grid1d(n), dim3(256), 0, st, d_solid, dist, n, d0, d_band);
-  return hipGetLastError();
+  if (band_bytes_dist(nx, ny, nz, d0))
+    return band_run<uint8_t>(d_solid, nx, ny, nz, d0, d_band, d_scratch, st);
+  return band_run<int>(d_solid, nx, ny, nz, d0, d_band, reinterpret_cast<int*>(d_scratch), st);
 }
 
 }  // namespace fs

@@ -145,6 +145,21 @@ def test_band_exact(fs, oracle, d0):
     assert np.array_equal(g, o)
 
 
+@pytest.mark.parametrize("d0", [200, 254, 255, 300])
+def test_band_deep_exact(fs, oracle, d0):
+    """Bands of hundreds of rings on a long thin grid (Manhattan diameter 299):
+    distances are kept in one byte up to 254 rings and in an int beyond."""
+    nx, ny, nz = 280, 12, 10
+    solid = np.zeros(nx * ny * nz, np.uint8)
+    solid.reshape(nz, ny, nx)[nz // 2, ny // 2, 3] = 1
+    g = fs.band(solid, nx, ny, nz, d0)
+    o = oracle.band(solid, nx, ny, nz, d0)
+    assert np.array_equal(g, o)
+    m = g.reshape(nz, ny, nx)
+    assert m[nz // 2, ny // 2, 4] == 1
+    assert m[nz // 2, ny // 2, nx - 1] == (1 if d0 >= (nx - 1 - 3) else 0)
+
+
 def test_band_no_solid(fs):
     assert fs.band(np.zeros(8 * 8 * 8, np.uint8), 8, 8, 8, 3).sum() == 0
 
