@@ -102,22 +102,26 @@ __global__ void __launch_bounds__(256) k_surface(const TriF* __restrict__ tris, 
   }
 }
 
-// exterior seeds: non-surface cells on the six faces
+// exterior seeds: non-surface cells on the six faces (cells are fewer than
+// 2^31, so the index arithmetic is 32-bit)
 __global__ void __launch_bounds__(256) k_seed(const uint8_t* __restrict__ surf, int nx, int ny,
                                               int nz, uint8_t* __restrict__ vis) {
-  const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
-  const size_t n = size_t(nx) * ny * nz;
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  const uint32_t n = uint32_t(nx) * uint32_t(ny) * uint32_t(nz);
   if (i >= n) return;
-  const int x = int(i % nx), y = int((i / nx) % ny), z = int(i / (size_t(nx) * ny));
+  const uint32_t row = i / uint32_t(nx);
+  const int x = int(i - row * uint32_t(nx));
+  const int y = int(row % uint32_t(ny)), z = int(row / uint32_t(ny));
   const bool face = x == 0 || y == 0 || z == 0 || x == nx - 1 || y == ny - 1 || z == nz - 1;
   vis[i] = (face && !surf[i]) ? 1 : 0;
 }
 
 // Sets the round's "something changed" flag.  In a round that fills the
 // exterior nearly every wavefront reaches new cells, and one atomic each on
-// the same word serialises them (measured: the first x sweep of a 50 M cell
-// grid took 0.74 ms, the round after it 0.07 ms); the flag only ever goes
-// 0 -> 1, so once a lane sees it set there is nothing to do.
+// the same word serialises them (with an unconditional atomic the first x
+// sweep of a 50 M cell grid took 0.74 ms, the next round, which sets
+// nothing, 0.07 ms); the flag only ever goes 0 -> 1, so once a lane sees it
+// set there is nothing to do.
 __device__ __forceinline__ void mark_changed(int* changed) {
   if (__hip_atomic_load(changed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
     atomicOr(changed, 1);
