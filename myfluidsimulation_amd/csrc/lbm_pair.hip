@@ -76,17 +76,18 @@ int pop_default_variant(const Geo& g) {
   // A workgroup walks its whole z-chunk, so a small grid gives too few of
   // them to fill 256 CUs: below 512 tiles of 64x4x16 (measured crossover,
   // about 128^3) the one-step kernel, with one workgroup per 256 cells, is
-  // faster.  64x8 tiles with 32-plane chunks re-read the fewest halo rows and
-  // planes and win once they still give >= 4 workgroups per CU (256^3: 46.6
-  // against 42.5 GLUPS; 768x256x256, 3072 of them: +4%); otherwise 64x4x16
-  // (the 384x128x128 channel gives 384 of the former).  On the largest grids
-  // 64x4x16 is ahead again (512^3, 8192 of the former: 43.7-44.0 against
-  // 42.5 GLUPS); the switch sits between the two measured sizes.
+  // faster.  Otherwise 64x4x16 tiles (three workgroups per CU), except where
+  // 64x8 tiles with 32-plane chunks (one per CU, the fewest halo rows and
+  // planes re-read) give 1024 to 2047 workgroups: at 256^3 (1024 of them)
+  // 64x4x16 leaves a third-full last round and 64x8x32 is ahead (46.0 against
+  // 42.9 GLUPS), while at 3072 (768x256x256 channel: 38.6 against 40.7) and
+  // 8192 (512^3: 42.5 against 43.7-44.0) 64x4x16 is; the 384x128x128
+  // channel gives only 384 of the former.
   const long cols = (g.nx + 63) / 64;
   const long tiles4 = cols * ((g.ny + 3) / 4) * ((g.nz + 15) / 16);
   if (tiles4 < 512) return kPopAA;
   const long tiles8z32 = cols * ((g.ny + 7) / 8) * ((g.nz + 31) / 32);
-  return (tiles8z32 >= 1024 && tiles8z32 < 6144) ? kPairT8Z32 : kPairT4Z16;
+  return (tiles8z32 >= 1024 && tiles8z32 < 2048) ? kPairT8Z32 : kPairT4Z16;
 }
 
 hipError_t launch_pop_pair(const Geo& g, const Phys& ph, const PairLaunch& a, bool hybrid,
