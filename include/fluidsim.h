@@ -74,8 +74,15 @@ typedef struct fs_solver fs_solver;
 /* ABI version (FS_ABI_VERSION) and the last error message of this thread. */
 int fs_version(void);
 const char* fs_last_error(void);
-/* Number of visible HIP devices (0 when none); never fails on a CPU host. */
+/* Number of visible HIP devices (0 when none).  Fails with FS_ERR_DEVICE,
+ * like every call that creates device state or takes a stream, when the
+ * process maps more than one HIP runtime or the one it maps is not the
+ * major.minor version the library was built against (see fs_hip_versions). */
 int fs_device_count(int* count);
+/* Extension (no reference counterpart): the HIP version the library was
+ * compiled against and the version of the runtime it is bound to, both as
+ * major*10000000 + minor*100000 + patch; either pointer may be NULL. */
+int fs_hip_versions(int* built_against, int* runtime);
 
 /* ===================== solver (the reference Solver concept) ============ */
 
@@ -196,6 +203,9 @@ int fs_batch_info(const fs_batch* b, int* ndomains, size_t* cells);
  * number of batch steps since creation (FS_ERR_STATE otherwise). */
 int fs_batch_set_flags(fs_batch* b, int domain, const uint8_t* host_flags);
 int fs_batch_set_inlet_profile(fs_batch* b, int domain, const float* host_ux);
+/* As fs_solver_set_bounce_rule, per domain (HOME domains may take the
+ * reference's pairing table, LBM3D_Home's rule, lbm3d_home.cu:27-30). */
+int fs_batch_set_bounce_rule(fs_batch* b, int domain, int rule);
 int fs_batch_reinit_from_macro(fs_batch* b, int domain, const float* d_rho,
                                const float* d_ux, const float* d_uy,
                                const float* d_uz);

@@ -45,6 +45,7 @@ SIGNATURES = {
     "fs_version": (_i, []),
     "fs_last_error": (C.c_char_p, []),
     "fs_device_count": (_i, [C.POINTER(_i)]),
+    "fs_hip_versions": (_i, [C.POINTER(_i), C.POINTER(_i)]),
     "fs_solver_create": (_i, [_i, _i, _i, _i, _f, _f, _f, _f, C.POINTER(_p)]),
     "fs_solver_destroy": (_i, [_p]),
     "fs_solver_dims": (_i, [_p, C.POINTER(_i), C.POINTER(_i), C.POINTER(_i), C.POINTER(_sz)]),
@@ -78,6 +79,7 @@ SIGNATURES = {
     "fs_batch_info": (_i, [_p, C.POINTER(_i), C.POINTER(_sz)]),
     "fs_batch_set_flags": (_i, [_p, _i, _p]),
     "fs_batch_set_inlet_profile": (_i, [_p, _i, _p]),
+    "fs_batch_set_bounce_rule": (_i, [_p, _i, _i]),
     "fs_batch_reinit_from_macro": (_i, [_p, _i, _p, _p, _p, _p]),
     "fs_batch_step": (_i, [_p, _i]),
     "fs_batch_timed_steps": (_i, [_p, _i, C.POINTER(_f)]),
@@ -166,6 +168,14 @@ def check(rc: int, where: str) -> None:
 
 def call(name: str, *args) -> None:
     check(getattr(lib(), name)(*args), name)
+
+
+def hip_versions():
+    """(HIP version the library was built against, runtime version it is
+    bound to), each major*10000000 + minor*100000 + patch"""
+    b, r = C.c_int(0), C.c_int(0)
+    call("fs_hip_versions", C.byref(b), C.byref(r))
+    return b.value, r.value
 
 
 def device_count() -> int:

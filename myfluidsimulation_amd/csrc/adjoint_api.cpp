@@ -148,6 +148,7 @@ int fs_adjoint_create(fs_solver* s, int T, int segment, const uint8_t* host_body
   if (!s) return fail(FS_ERR_INVALID, "null solver handle");
   if (T < 1) return fail(FS_ERR_INVALID, "T must be >= 1");
   if (segment < 0 || segment > T) return fail(FS_ERR_INVALID, "segment must be in [0, T]");
+  if (int rc = fs::runtime_check()) return rc;
   const fs::Geo& g = s->g;
   for (uint32_t c = 0; c < g.n; ++c)
     if ((s->hflags[c] & FS_FLAG_INLET) && (c % uint32_t(g.nx)) != 0)

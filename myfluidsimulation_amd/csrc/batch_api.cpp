@@ -220,6 +220,7 @@ int fs_batch_create(const fs_domain_desc* descs, int ndomains, fs_batch** out) {
   *out = nullptr;
   if (!descs || ndomains < 1 || ndomains > 65535)
     return fail(FS_ERR_INVALID, "need 1..65535 domain descriptors");
+  if (int rc = fs::runtime_check()) return rc;
   try {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
@@ -463,6 +464,26 @@ int fs_batch_set_inlet_profile(fs_batch* b, int d, const float* host_ux) {
     FS_HIP(hipMemcpyAsync(D.inlet, host_ux, bytes, hipMemcpyHostToDevice, b->stream));
   else
     FS_HIP(hipMemsetAsync(D.inlet, 0, bytes, b->stream));
+  FS_HIP(hipStreamSynchronize(b->stream));
+  return FS_OK;
+}
+
+int fs_batch_set_bounce_rule(fs_batch* b, int d, int rule) {
+  if (int rc = check_dom(b, d)) return rc;
+  if (rule != FS_BOUNCE_HALFWAY && rule != FS_BOUNCE_REFERENCE_TABLE)
+    return fail(FS_ERR_INVALID, "unknown bounce rule %d", rule);
+  const Dom& D = b->doms[size_t(d)];
+  if (D.home < 0) {
+    if (rule == FS_BOUNCE_REFERENCE_TABLE)
+      return fail(FS_ERR_STATE,
+                  "the reference pairing table is not a permutation of the populations; only "
+                  "HOME's gather can use it");
+    return FS_OK;
+  }
+  // the kernels read the rule from the device descriptor table on every launch
+  b->homeTab[size_t(D.home)].ph.bounce = rule;
+  FS_HIP(hipMemcpyAsync(b->d_home + D.home, &b->homeTab[size_t(D.home)],
+                        sizeof(fs::BatchHomeDomain), hipMemcpyHostToDevice, b->stream));
   FS_HIP(hipStreamSynchronize(b->stream));
   return FS_OK;
 }

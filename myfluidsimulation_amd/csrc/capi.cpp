@@ -4,7 +4,10 @@
 // the device, converts HIP errors and C++ exceptions into status codes and a
 // per-thread message, and sequences the kernels of lbm_pop.hip, lbm_home.hip,
 // fields.hip and geom.hip on the solver's stream.
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
+#include <hip/hip_version.h>
+#include <link.h>
 
 #include <algorithm>
 #include <cmath>
@@ -67,7 +70,51 @@ bool fault_injected(const char* what) {
   return v && std::strstr(v, what) != nullptr;
 }
 
+// Every loaded object whose file name starts with libamdhip64.so (the HIP
+// runtime; a process can map two copies under different names, e.g. one
+// bundled with another package beside the system one).
+int collect_hip_runtime(struct dl_phdr_info* info, size_t, void* data) {
+  const char* name = info->dlpi_name;
+  if (!name || !*name) return 0;
+  const char* base = std::strrchr(name, '/');
+  base = base ? base + 1 : name;
+  if (std::strncmp(base, "libamdhip64.so", 14) == 0)
+    static_cast<std::vector<std::string>*>(data)->push_back(name);
+  return 0;
+}
+
 }  // namespace
+
+// The library must run on the HIP runtime it was compiled against, and alone:
+// kernels, streams and device pointers of one runtime are meaningless to a
+// second runtime mapped into the same process.  Checked on every entry point
+// that creates device state or accepts a foreign stream.
+int fs::runtime_check() {
+  std::vector<std::string> rts;
+  dl_iterate_phdr(collect_hip_runtime, &rts);
+  if (rts.size() > 1) {
+    std::string all;
+    for (const std::string& r : rts) all += (all.empty() ? "" : ", ") + r;
+    return fail(FS_ERR_DEVICE,
+                "%zu HIP runtimes are mapped into this process (%s); libfluidsim refuses to run "
+                "beside a second runtime (load only one, e.g. do not import a package that "
+                "bundles its own HIP runtime into a process using libfluidsim)",
+                rts.size(), all.c_str());
+  }
+  int v = 0;
+  if (hipRuntimeGetVersion(&v) != hipSuccess)
+    return fail(FS_ERR_DEVICE, "hipRuntimeGetVersion failed");
+  if (v / 100000 != HIP_VERSION / 100000) {
+    Dl_info di{};
+    const char* where = "?";
+    if (dladdr(reinterpret_cast<void*>(&hipRuntimeGetVersion), &di) && di.dli_fname)
+      where = di.dli_fname;
+    return fail(FS_ERR_DEVICE,
+                "libfluidsim was built against HIP %d.%d but is bound to HIP runtime %d.%d (%s)",
+                HIP_VERSION_MAJOR, HIP_VERSION_MINOR, v / 10000000, (v / 100000) % 100, where);
+  }
+  return FS_OK;
+}
 
 namespace {
 
@@ -261,9 +308,20 @@ const char* fs_last_error(void) { return g_err.c_str(); }
 
 int fs_device_count(int* count) {
   if (!count) return fail(FS_ERR_INVALID, "null count");
+  *count = 0;
+  if (int rc = fs::runtime_check()) return rc;
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
   *count = n;
+  return FS_OK;
+}
+
+int fs_hip_versions(int* built_against, int* runtime) {
+  if (built_against) *built_against = HIP_VERSION;
+  if (runtime) {
+    *runtime = 0;
+    FS_HIP(hipRuntimeGetVersion(runtime));
+  }
   return FS_OK;
 }
 
@@ -280,6 +338,7 @@ int fs_solver_create(int method, int nx, int ny, int nz, float tau, float fx, fl
     if (!(tau > 0.5f) || !std::isfinite(tau))
       return fail(FS_ERR_INVALID, "tau must be finite and > 0.5");
     if (fault_injected("alloc")) return fail(FS_ERR_ALLOC, "injected allocation failure");
+    if (int rc = fs::runtime_check()) return rc;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
       return fail(FS_ERR_DEVICE, "no HIP device available");
@@ -452,6 +511,7 @@ int fs_solver_step_async(fs_solver* s, int nsteps) {
 
 int fs_solver_set_stream(fs_solver* s, void* st) {
   if (int rc = check_solver(s)) return rc;
+  if (int rc = fs::runtime_check()) return rc;
   FS_HIP(hipStreamSynchronize(s->stream));
   s->stream = st ? reinterpret_cast<hipStream_t>(st) : s->own;
   return FS_OK;
@@ -678,11 +738,12 @@ int fs_voxelize(const float* tri_xyz, int ntri, const float* bbmin3, const float
   if (nx < 1 || ny < 1 || nz < 1 || !mask_out || !bbmin3 || !bbmax3 || ntri < 0 ||
       (ntri > 0 && !tri_xyz))
     return fail(FS_ERR_INVALID, "bad voxelize arguments");
+  if (int rc = fs::runtime_check()) return rc;
   return guarded([&]() -> int {
     const int r = refine < 1 ? 1 : refine;
     const size_t nf = size_t(nx) * r * size_t(ny) * r * size_t(nz) * r;
-    if (size_t(nx) * r >= (1u << 31) || nf >= (size_t(1) << 40))
-      return fail(FS_ERR_INVALID, "refined grid too large");
+    if (size_t(nx) * r >= (1u << 31) || nf >= (size_t(1) << 31))
+      return fail(FS_ERR_INVALID, "refined grid of %zu cells exceeds 2^31", nf);
     // compute_fit_transform + make_transformed_tris (voxelize.cpp:83-113)
     float bb[3];
     for (int k = 0; k < 3; ++k) bb[k] = bbmax3[k] - bbmin3[k];
@@ -732,6 +793,8 @@ int fs_band(const uint8_t* solid, int nx, int ny, int nz, int d0, uint8_t* band_
   if (!solid || !band_out || nx < 1 || ny < 1 || nz < 1)
     return fail(FS_ERR_INVALID, "bad band arguments");
   const size_t n = size_t(nx) * ny * nz;
+  if (n >= (size_t(1) << 31)) return fail(FS_ERR_INVALID, "grid of %zu cells exceeds 2^31", n);
+  if (int rc = fs::runtime_check()) return rc;
   uint8_t* d_solid = nullptr;
   uint8_t* d_band = nullptr;
   uint8_t* d_scr = nullptr;
@@ -848,6 +911,7 @@ int fs_device_malloc(void** p, size_t bytes) {
   if (!p) return fail(FS_ERR_INVALID, "null output");
   *p = nullptr;
   if (fault_injected("alloc")) return fail(FS_ERR_ALLOC, "injected allocation failure");
+  if (int rc = fs::runtime_check()) return rc;
   hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess) return fail(FS_ERR_ALLOC, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
   return FS_OK;

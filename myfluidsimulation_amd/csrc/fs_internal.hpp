@@ -270,6 +270,9 @@ hipError_t launch_fill(float* p, float v, size_t n, hipStream_t st);
 
 // error reporting shared by the C ABI sources (capi.cpp, batch_api.cpp)
 int fail(int code, const char* fmt, ...);
+// FS_ERR_DEVICE (with a message) unless exactly one HIP runtime is mapped
+// into the process and its major.minor version is the one compiled against
+int runtime_check();
 
 // plain streaming copy (bandwidth ceiling probe)
 hipError_t launch_copy_probe(void* dst, const void* src, size_t bytes, hipStream_t st);

@@ -128,3 +128,66 @@ def test_cli_reports_missing_device_and_stl(tmp_path):
     assert r.returncode == 1
     assert "STL の読み込みに失敗しました" in r.stderr
     assert "[error]" in r.stderr
+
+
+def _torch_hip_version():
+    """major.minor of the HIP runtime the installed torch bundles, read from
+    its version.py without importing torch (None when torch is absent or a
+    CPU build)"""
+    import importlib.util
+
+    spec = importlib.util.find_spec("torch")
+    if spec is None or not spec.origin:
+        return None
+    txt = open(os.path.join(os.path.dirname(spec.origin), "version.py")).read()
+    m = re.search(r"^hip\b.*?=\s*'(\d+)\.(\d+)", txt, re.M)
+    return (int(m.group(1)), int(m.group(2))) if m else None
+
+
+_RUNTIME_PROBE = r"""
+import ctypes as C, sys
+order = sys.argv[1]
+def load():
+    L = C.CDLL(sys.argv[2])
+    L.fs_last_error.restype = C.c_char_p
+    return L
+if order == "torch_first":
+    import torch  # noqa: F401
+    L = load()
+else:
+    L = load()
+    import torch  # noqa: F401
+n = C.c_int(-1)
+rc = L.fs_device_count(C.byref(n))
+print(rc)
+print(L.fs_last_error().decode())
+"""
+
+
+@pytest.mark.parametrize("order", ["torch_first", "library_first"])
+def test_foreign_hip_runtime_is_refused(order):
+    """A process that imports torch maps torch's bundled HIP runtime: imported
+    first, libfluidsim binds to it (same soname); imported after, a second
+    runtime is mapped beside the one libfluidsim loaded.  The library refuses
+    both unless the bundled runtime is the major.minor it was built against
+    (and even then refuses two mapped runtimes)."""
+    tv = _torch_hip_version()
+    if tv is None:
+        pytest.skip("no ROCm torch installed")
+    built, _ = capi.hip_versions()
+    same = tv == (built // 10000000, (built // 100000) % 100)
+    r = subprocess.run(["python", "-c", _RUNTIME_PROBE, order, capi.LIB_PATH],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    rc, msg = r.stdout.strip().split("\n", 1) if "\n" in r.stdout.strip() else (r.stdout.strip(), "")
+    if order == "library_first":
+        assert int(rc) == capi.FS_ERR_DEVICE and "HIP runtimes are mapped" in msg
+    elif same:
+        assert int(rc) == capi.FS_OK
+    else:
+        assert int(rc) == capi.FS_ERR_DEVICE and "built against HIP" in msg
+
+
+def test_hip_versions_reported():
+    built, runtime = capi.hip_versions()
+    assert built // 100000 == runtime // 100000  # this process maps only the build's runtime
