@@ -11,7 +11,8 @@
 // 200x150x120 tau 0.58 domain and voxel placement (scale 0.5 about the
 // centre), the same stdout lines and CSV columns and row format
 // (main_compare.cpp:304-309,384,468,515,630-698).  Extensions: --nx --ny --nz
-// --tau.  The ms_per_step_* columns time each configuration alone on a
+// --tau, --bounce halfway (HOME bounce-back in the opposite direction instead
+// of the reference's pairing table).  The ms_per_step_* columns time each configuration alone on a
 // single-domain solver over min(steps, 50) steps (the batch steps them all
 // at once, so it has no per-configuration time of its own).
 #include <algorithm>
@@ -38,6 +39,9 @@ struct SweepConfig {
   int hybridBand = 2;
   int sweepMax = -1;
   int sweepSteps = 500;
+  // HOME's bounce-back partner: the reference's pairing table, as LBM3D_Home
+  // (extension: --bounce halfway)
+  int homeBounce = FS_BOUNCE_REFERENCE_TABLE;
 };
 
 void parse(int argc, char** argv, SweepConfig& c) {
@@ -75,6 +79,9 @@ void parse(int argc, char** argv, SweepConfig& c) {
     else if (a == "--ny" && left >= 1) c.Ny = std::max(8, I());
     else if (a == "--nz" && left >= 1) c.Nz = std::max(1, I());
     else if (a == "--tau" && left >= 1) c.tau = std::max(0.51f, F());
+    else if (a == "--bounce" && left >= 1)
+      c.homeBounce = std::string(argv[++i]) == "halfway" ? FS_BOUNCE_HALFWAY
+                                                          : FS_BOUNCE_REFERENCE_TABLE;
   }
   if (c.amrFactor < 1) c.amrFactor = 1;
   c.amrThreshold = std::clamp(c.amrThreshold, 0.0f, 1.0f);
@@ -111,13 +118,14 @@ Stats stats(const std::vector<float>& rho, const std::vector<float>& u, const st
 }
 
 double solo_ms_per_step(int method, const fs_domain_desc& dd, const std::vector<unsigned char>& fl,
-                        const float* d_rho, int steps) {
+                        const float* d_rho, int steps, int homeBounce) {
   if (steps <= 0) return 0.0;
   fs_solver* s = nullptr;
   fs_throw_if(fs_solver_create(method, dd.nx, dd.ny, dd.nz, dd.tau, dd.fx, dd.fy, dd.fz, &s),
               "solver_create");
   float ms = 0.0f;
-  int rc = fs_solver_set_flags(s, fl.data());
+  int rc = method == FS_METHOD_HOME ? fs_solver_set_bounce_rule(s, homeBounce) : FS_OK;
+  if (rc == FS_OK) rc = fs_solver_set_flags(s, fl.data());
   if (rc == FS_OK) rc = fs_solver_reinit_from_macro(s, d_rho, nullptr, nullptr, nullptr);
   if (rc == FS_OK) rc = fs_solver_timed_steps(s, steps, &ms);
   fs_solver_destroy(s);
@@ -226,6 +234,7 @@ int run(const SweepConfig& c) {
     fs_batch* b;
     ~BatchGuard() { fs_batch_destroy(b); }
   } guard{b};
+  fs_throw_if(fs_batch_set_bounce_rule(b, 1, c.homeBounce), "batch_set_bounce_rule");
   for (int k = 0; k < nd; ++k) {
     fs_throw_if(fs_batch_set_flags(b, k, flags[k].data()), "batch_set_flags");
     fs_throw_if(fs_batch_reinit_from_macro(b, k, init.rho, nullptr, nullptr, nullptr),
@@ -246,8 +255,10 @@ int run(const SweepConfig& c) {
   fetch(1);
   const Stats H = stats(r, u, v, w, near, solid, nFluid, &baseline, nullptr);
   const int timedSteps = std::min(c.sweepSteps, 50);
-  const double msL = solo_ms_per_step(FS_METHOD_BGK, descs[0], flags[0], init.rho, timedSteps);
-  const double msH = solo_ms_per_step(FS_METHOD_HOME, descs[1], flags[1], init.rho, timedSteps);
+  const double msL =
+      solo_ms_per_step(FS_METHOD_BGK, descs[0], flags[0], init.rho, timedSteps, c.homeBounce);
+  const double msH =
+      solo_ms_per_step(FS_METHOD_HOME, descs[1], flags[1], init.rho, timedSteps, c.homeBounce);
 
   std::printf(
       "d0,legacy_cells,legacy_ratio,mean_u_legacy,mean_u_home,mean_u_hybrid,"
@@ -258,7 +269,8 @@ int run(const SweepConfig& c) {
     const int k = 2 + d0;
     fetch(k);
     const Stats Y = stats(r, u, v, w, near, solid, nFluid, &baseline, nullptr);
-    const double msY = solo_ms_per_step(FS_METHOD_HYBRID, descs[k], flags[k], init.rho, timedSteps);
+    const double msY =
+        solo_ms_per_step(FS_METHOD_HYBRID, descs[k], flags[k], init.rho, timedSteps, c.homeBounce);
     const double ratio = nFluid > 0 ? double(nLeg[k]) / double(nFluid) : 0.0;
     std::printf(
         "%d,%d,%.6f,%.9e,%.9e,%.9e,%.9e,%.9e,%.9e,%.9e,%.9e,%.9e,%.9e,%.6f,%.6f,%.6f,%d,%.6f,%d,"

@@ -12,6 +12,8 @@
 //   --walls yz               solid wall planes at y=0,ny-1 and z=0,nz-1
 //   --vtk-format binary      VTK files in the legacy BINARY encoding
 //                            (big-endian float32) instead of ASCII
+//   --bounce halfway         HOME only: bounce back in the opposite direction
+//                            instead of the reference's pairing table
 #pragma once
 
 #include <algorithm>
@@ -65,6 +67,7 @@ struct RunConfig {
   bool outlet = false;
   bool wallsYZ = false;
   bool vtkBinary = false;
+  bool bounceHalfway = false;  // HOME: opposite-direction bounce-back
 };
 
 struct HybridConfig {
@@ -137,6 +140,8 @@ inline void parse_common_args(int argc, char** argv, DomainConfig& dom, Obstacle
       run.wallsYZ = std::string(argv[++i]) == "yz";
     } else if (a == "--vtk-format" && left >= 1) {
       run.vtkBinary = std::string(argv[++i]) == "binary";
+    } else if (a == "--bounce" && left >= 1) {
+      run.bounceHalfway = std::string(argv[++i]) == "halfway";
     }
   }
   dom.Nx = std::max(dom.Nx, 8);

@@ -8,6 +8,7 @@
 #pragma once
 
 #include <cstdio>
+#include <type_traits>
 #include <vector>
 
 #include "sim_common.hpp"
@@ -69,6 +70,8 @@ inline Domain to_domain(const DomainConfig& c) {
 template <class Solver>
 void setup_solver(Solver& solver, const DomainConfig& domCfg, const RunConfig& runCfg,
                   std::vector<unsigned char>& solid, const std::vector<unsigned char>* band) {
+  if constexpr (std::is_same_v<Solver, LBM3D_Home>)
+    if (runCfg.bounceHalfway) solver.setBounceRule(FS_BOUNCE_HALFWAY);
   bool special = false;
   std::vector<unsigned char> fl = build_flags(domCfg, runCfg, solid, special);
   if (band)

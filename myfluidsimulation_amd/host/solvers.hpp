@@ -87,7 +87,21 @@ class FsSolver {
 };
 
 class LBM3D_Legacy : public FsSolver<FS_METHOD_BGK> {};
-class LBM3D_Home : public FsSolver<FS_METHOD_HOME> {};
+
+// HOME gathers, so the reference's own bounce-back pairing table (opp19,
+// lbm3d_home.cu:27-30, used at :219-226) gives a deterministic update: the
+// drop-in class uses it, as the reference does.  setBounceRule
+// (FS_BOUNCE_HALFWAY) switches to the mass-conserving opposite direction.
+class LBM3D_Home : public FsSolver<FS_METHOD_HOME> {
+ public:
+  void init(const Domain& d) {
+    FsSolver::init(d);
+    setBounceRule(FS_BOUNCE_REFERENCE_TABLE);
+  }
+  void setBounceRule(int rule) {
+    fs_throw_if(fs_solver_set_bounce_rule(s_, rule), "setBounceRule");
+  }
+};
 
 class LBM3D_Hybrid : public FsSolver<FS_METHOD_HYBRID> {
  public:

@@ -45,7 +45,10 @@ def case_args(stl, scale, trans, vtk_dir, extra=()):
              "--vtk-dir", vtk_dir] + list(extra))
 
 
-def replay(oracle, method, stl, scale, trans, out_dir, tag, channel=False, d0=None, inlet_u=0.04):
+def replay(oracle, method, stl, scale, trans, out_dir, tag, channel=False, d0=None, inlet_u=0.04,
+           bounce=None):
+    """the CLI run on the oracle; HOME's bounce rule defaults to the
+    reference's pairing table, as fluidsim_home's does"""
     tris, bmin, bmax = oracle.load_stl(stl)
     solid = oracle.voxelize(tris, bmin, bmax, NX, NY, NZ, scale, trans)
     if channel:
@@ -56,6 +59,8 @@ def replay(oracle, method, stl, scale, trans, out_dir, tag, channel=False, d0=No
         flags = flags | (oracle.band((flags & 1).astype(np.uint8), NX, NY, NZ, d0) * 2).astype(
             np.uint8)
     o = oracle.Oracle(method, NX, NY, NZ, np.float32(TAU), (np.float32(FX), 0.0, 0.0))
+    if method == 1:
+        o.set_bounce(1 if bounce is None else bounce)
     o.set_flags(flags)
     n = NX * NY * NZ
     rho = oracle.rho_slab(NX, NY, NZ, SLAB["axis"], SLAB["amp"], SLAB["center"], SLAB["width"])
@@ -113,6 +118,25 @@ def test_single_solver_cli(oracle, obstacle, tmp_path, name, method, tag):
     want = str(tmp_path / "oracle")
     replay(oracle, method, stl, scale, trans, want, tag)
     compare_dirs(got, want)
+
+
+def test_home_cli_bounce_rules(oracle, obstacle, tmp_path):
+    """fluidsim_home defaults to the reference's deterministic pairing table
+    (lbm3d_home.cu:27-30,219-226); --bounce halfway selects the opposite
+    direction.  Both runs equal their oracle replays, and the two rules give
+    different files on this obstacle (so each comparison sees its rule)."""
+    stl, scale, trans = obstacle
+    outs = {}
+    for rule, extra in ((1, []), (0, ["--bounce", "halfway"])):
+        got = str(tmp_path / f"gpu{rule}")
+        run_cli("fluidsim_home", case_args(stl, scale, trans, got, extra), str(tmp_path))
+        want = str(tmp_path / f"oracle{rule}")
+        replay(oracle, 1, stl, scale, trans, want, "home", bounce=rule)
+        compare_dirs(got, want)
+        last = f"vel_home_{STEPS // EVERY * EVERY:06d}.vtk"
+        with open(os.path.join(got, last), "rb") as fh:
+            outs[rule] = fh.read()
+    assert outs[0] != outs[1]
 
 
 def test_hybrid_cli(oracle, obstacle, tmp_path):
