@@ -314,6 +314,9 @@ class Batch:
             a = np.ascontiguousarray(np.asarray(ux, np.float32).reshape(-1))
             call("fs_batch_set_inlet_profile", self.h, int(d), _host_ptr(a))
 
+    def set_bounce_rule(self, d, rule):
+        call("fs_batch_set_bounce_rule", self.h, int(d), int(rule))
+
     def reinit_from_macro(self, d, rho=None, ux=None, uy=None, uz=None):
         keep = []
 
@@ -393,6 +396,53 @@ class Adjoint:
 
     def backward_ptrs(self, *ptrs):
         call("fs_adjoint_backward", self.h, *ptrs)
+
+    # -- host (numpy) front end: staging buffers on the device, no torch
+    def _stage(self, arrays):
+        bufs = [None if a is None else
+                DeviceBuffer.from_host(np.ascontiguousarray(np.asarray(a, np.float32).reshape(-1)))
+                for a in arrays]
+        return bufs, [None if b is None else C.c_void_p(b.ptr) for b in bufs]
+
+    def forward(self, rho0=None, ux0=None, uy0=None, uz0=None, inlet=None):
+        """T steps from (rho0, u0) (N host floats each, None: rho 1 / u 0)
+        with the inlet profile (ny*nz floats, None keeps the solver's).
+        Returns (rho, ux, uy, uz, drag) as numpy arrays (N, N, N, N, T)."""
+        n, s = self.solver.n, self.solver
+        ins, iptr = self._stage([rho0, ux0, uy0, uz0, inlet])
+        outs = [DeviceBuffer(4 * n) for _ in range(4)] + [DeviceBuffer(4 * self.steps)]
+        try:
+            call("fs_adjoint_forward", self.h, *iptr, *[C.c_void_p(b.ptr) for b in outs])
+            s.synchronize()
+            res = [b.to_host(np.float32, n) for b in outs[:4]]
+            res.append(outs[4].to_host(np.float32, self.steps))
+        finally:
+            for b in ins + outs:
+                if b is not None:
+                    b.free()
+        return tuple(res)
+
+    def backward(self, g_rho=None, g_ux=None, g_uy=None, g_uz=None, g_drag=None,
+                 want=(True, True, True, True, True)):
+        """Gradients of the last forward pass, given dL/d(end macros) (N host
+        floats each) and dL/d(drag) (T floats), any None for zero.  Returns
+        (dL/drho0, dL/dux0, dL/duy0, dL/duz0, dL/dinlet) as numpy arrays, None
+        where `want` is False."""
+        n, s = self.solver.n, self.solver
+        ins, iptr = self._stage([g_rho, g_ux, g_uy, g_uz, g_drag])
+        sizes = [n, n, n, n, s.ny * s.nz]
+        outs = [DeviceBuffer(4 * m) if w else None for m, w in zip(sizes, want)]
+        try:
+            call("fs_adjoint_backward", self.h, *iptr,
+                 *[None if b is None else C.c_void_p(b.ptr) for b in outs])
+            s.synchronize()
+            res = tuple(None if b is None else b.to_host(np.float32, m)
+                        for b, m in zip(outs, sizes))
+        finally:
+            for b in ins + outs:
+                if b is not None:
+                    b.free()
+        return res
 
     def close(self):
         if getattr(self, "h", None):

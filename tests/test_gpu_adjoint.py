@@ -1,239 +1,162 @@
 """GPU: gradients of T-step runs from the adjoint kernels (fs_adjoint_*,
-through the torch.autograd.Function) against FP64 torch autograd of the
-same update rules (tests/torch_lbm.py, pinned against the FP64 oracle by
-tests/test_torch_ref.py) and against central finite differences of the FP64
-forward on 32 random directions.
+through the C ABI with host staging; no torch in this process) against the
+FP64 reference gradients of tests/golden/adjoint32.npz, which
+tests/golden/make_adjoint_golden.py computed on the CPU with torch autograd
+of the FP64 formulation tests/torch_lbm.py (pinned against the FP64 oracle by
+tests/test_torch_ref.py), and against the FP64 central differences on 32
+seeded random directions stored with them.
 
-Objectives (32^3, FP32 kernels):
-  (a) Taylor-Green inverse problem: J = 1/2 |u(T) - u_target|^2, the target
+Objectives (32^3, T = 40, FP32 kernels):
+  (a) Taylor-Green inverse problem: J = 1/2 |u(T) - target|^2, the target
       the end field of a run from the vortex plus 8 seeded Fourier modes
       (amplitude 0.01); gradient with respect to the initial velocity.
-  (b) channel with a seeded icosphere: J = mean over steps of the drag on
-      the obstacle; gradient with respect to the inlet profile.
+  (b) channel with a seeded icosphere: J = mean over the steps of the drag
+      on the obstacle; gradient with respect to the inlet profile.
 Tolerances: relative L2 error of the gradient against FP64 autograd
 <= 1e-4; every directional derivative within 1e-4 of the central
-difference, relative to max(|central difference|, |grad| / sqrt(dim)).
+difference, relative to max(|central difference|, |grad| / sqrt(dim)) (a
+direction nearly orthogonal to the gradient makes the difference itself
+close to zero, and a plain relative error then measures only rounding).
 """
+import os
+import sys
+
 import numpy as np
 import pytest
 
-from myfluidsimulation_amd import workloads as wl
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, "golden"))
+import make_adjoint_golden as mg  # noqa: E402  (direction_set and the case constants)
 
-# torch (and the torch formulation) are imported inside the tests only, so
-# that collecting this module keeps torch out of the GPU test process
-torch = TorchLBM = None
-HOME, HYBRID = 1, 2
+pytestmark = pytest.mark.gpu
 
-pytestmark = [
-    pytest.mark.gpu,
-    pytest.mark.skip(reason=(
-        "off the GPU until the cause of the GPU memory fault seen in the first run of "
-        "test_tgv_inverse_gradient[0] (torch and libfluidsim in one process, torch FP64 "
-        "reference kernels before the adjoint kernels) is shown; the adjoint kernels pass "
-        "this module against FP64 autograd in a CPU build of the library")),
-]
-
-N = 32
-T = 40
-DEV = "cuda"
+G = np.load(os.path.join(HERE, "golden", "adjoint32.npz"))
+N, T, TAU = int(G["N"]), int(G["T"]), float(G["tau"])
+METHODS = [name for name, _ in mg.METHODS]
+CODE = dict(mg.METHODS)
 
 
-@pytest.fixture(autouse=True)
-def _torch():
-    global torch, DEV, TorchLBM
-    import torch as t
-    from torch_lbm import TorchLBM as R
-
-    torch, TorchLBM = t, R
-    DEV = "cuda" if t.cuda.is_available() else "cpu"
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64).reshape(-1)
+    b = np.asarray(b, np.float64).reshape(-1)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def tgv_case(fs, method):
-    rho, ux, uy, uz = wl.taylor_green_3d(N, N, N, 0.05)
-    px, py, pz = wl.fourier_modes(N, N, N, 8, 0.01, seed=11)
-    flags = np.zeros(N ** 3, np.uint8)  # Hybrid: no band, every cell relaxes at omega_S
-    return dict(tau=0.6, flags=flags, body=None, rho=rho, u=(ux, uy, uz),
-                u_true=(ux + px, uy + py, uz + pz), inlet=np.zeros(N * N, np.float32))
-
-
-def channel_case(fs, method, tmp_path):
-    path = str(tmp_path / "obstacle.stl")
-    scale, trans = wl.obstacle_stl(path, N, N, N, (10.0, 16.0, 16.0), 5.0, seed=77)
-    tris, bmin, bmax = fs.load_stl(path)
-    solid = fs.voxelize(tris, bmin, bmax, N, N, N, scale, trans)
-    flags = wl.channel_flags(N, N, N, solid)
-    if method == HYBRID:
-        flags = flags | (fs.band((flags & 1).astype(np.uint8), N, N, N, 2) * 2).astype(np.uint8)
-    rng = np.random.default_rng(5)
-    y = (np.arange(N) + 0.5) / N
-    prof = 0.05 * 4 * y * (1 - y)
-    inlet = (prof[None, :] * prof[:, None] * 4 + 0.002 * rng.standard_normal((N, N)))
-    rho, ux, uy, uz = wl.uniform_flow(N ** 3, 0.03)
-    return dict(tau=0.6, flags=flags, body=solid.astype(np.uint8), rho=rho, u=(ux, uy, uz),
-                inlet=inlet.astype(np.float32).reshape(-1))
-
-
-def make_solver(fs, method, c):
-    s = fs.Solver(method, N, N, N, c["tau"])
-    s.set_flags(c["flags"])
-    if method == HYBRID:
-        s.set_omegas(1 / c["tau"], 1.25, 1.4)
-    return s
-
-
-def torch_ref(method, c):
-    om = (1 / c["tau"], 1.25, 1.4) if method == HYBRID else None
-    return TorchLBM(method, N, N, N, c["tau"], (0, 0, 0), c["flags"], omegas=om,
-                    body=c["body"], device=DEV)
-
-
-def d64(a, shape):
-    return torch.tensor(np.asarray(a, np.float64), device=DEV).reshape(shape)
-
-
-def f32(a):
-    return torch.tensor(np.asarray(a, np.float32), device=DEV)
-
-
-def objective_a(ref, c, target):
-    sh = (N, N, N)
-    rho0 = d64(c["rho"], sh)
-    uin = d64(c["inlet"], (N, N))
-
-    def J(u0):
-        _, uT, _ = ref.run(rho0, u0, uin, T)
-        return 0.5 * ((uT - target) ** 2).sum()
-    return J
-
-
-def objective_b(ref, c):
-    sh = (N, N, N)
-    rho0 = d64(c["rho"], sh)
-    u0 = torch.stack([d64(a, sh) for a in c["u"]])
-
-    def J(uin):
-        _, _, drag = ref.run(rho0, u0, uin, T)
-        return drag.mean()
-    return J
-
-
-def check_fd(J, x0, grad_ad, ndir=32, seed=0):
-    """Central differences of the FP64 objective on ndir random unit
-    directions v against <grad_ad, v>.  The error of each directional
-    derivative is taken relative to max(|fd|, |grad| / sqrt(dim)), the
-    second being the typical size of a projection on a random unit
-    direction: a direction nearly orthogonal to the gradient makes fd itself
-    close to zero, and a plain relative error then measures only rounding."""
-    g = torch.Generator(device="cpu").manual_seed(seed)
-    rms = float(grad_ad.double().norm()) / np.sqrt(x0.numel())
+def check_directions(grad, fd, shape):
+    """<grad, v> against the stored central differences along the same
+    seeded directions"""
+    g = np.asarray(grad, np.float64).reshape(shape)
+    rms = float(np.linalg.norm(g)) / np.sqrt(g.size)
     worst = 0.0
-    for _ in range(ndir):
-        v = torch.randn(x0.shape, generator=g, dtype=torch.float64).to(DEV)
-        v /= v.norm()
-        eps = 1e-5 * max(1.0, float(x0.abs().max()))
-        with torch.no_grad():
-            fd = (J(x0 + eps * v) - J(x0 - eps * v)).item() / (2 * eps)
-        ad = float((grad_ad.double() * v).sum())
-        rel = abs(ad - fd) / max(abs(fd), rms)
+    for v, d in zip(mg.direction_set(shape, int(G["ndir"]), int(G["dir_seed"])), fd):
+        ad = float((g * v).sum())
+        rel = abs(ad - d) / max(abs(d), rms)
         worst = max(worst, rel)
-        assert rel <= 1e-4, f"directional derivative {ad} vs central difference {fd}"
+        assert rel <= 1e-4, f"directional derivative {ad} vs central difference {d}"
     return worst
 
 
-@pytest.mark.parametrize("method", [0, HOME, HYBRID])
-def test_tgv_inverse_gradient(fs, method):
-    from myfluidsimulation_amd.adjoint import DifferentiableRun
-
-    c = tgv_case(fs, method)
-    sh = (N, N, N)
-    ref = torch_ref(method, c)
-    with torch.no_grad():
-        _, target, _ = ref.run(d64(c["rho"], sh), torch.stack([d64(a, sh) for a in c["u_true"]]),
-                               d64(c["inlet"], (N, N)), T)
-    s = make_solver(fs, method, c)
-    run = DifferentiableRun(s, T)
-    u0 = [f32(a).requires_grad_() for a in c["u"]]
-    _, ux, uy, uz, _ = run(*u0, rho0=f32(c["rho"]))
-    tg = target.float().reshape(3, -1)
-    loss = 0.5 * (((ux - tg[0]) ** 2).sum() + ((uy - tg[1]) ** 2).sum() + ((uz - tg[2]) ** 2).sum())
-    loss.backward()
-    g_ad = torch.stack([u.grad for u in u0]).reshape(3, *sh)
-
-    J = objective_a(ref, c, target)
-    x0 = torch.stack([d64(a, sh) for a in c["u"]]).requires_grad_()
-    Jx = J(x0)
-    Jx.backward()
-    assert float(loss.detach()) == pytest.approx(float(Jx.detach()), rel=1e-4)
-    err = float((g_ad.double() - x0.grad).norm() / x0.grad.norm())
-    print(f"method {method}: |g| {float(x0.grad.norm()):.4e}  rel L2 err vs FP64 autograd {err:.2e}")
-    assert err <= 1e-4
-    worst = check_fd(J, x0.detach(), g_ad)
-    print(f"method {method}: worst directional rel err vs central differences {worst:.2e}")
-    run.close()
-    s.close()
+def make_solver(fs, name, flags):
+    s = fs.Solver(CODE[name], N, N, N, TAU)
+    s.set_flags(flags)
+    if name == "hybrid":
+        s.set_omegas(*[float(w) for w in G["hyb_omegas"]])
+    return s
 
 
-@pytest.mark.parametrize("method", [0, HOME, HYBRID])
-def test_channel_drag_gradient(fs, method, tmp_path):
-    from myfluidsimulation_amd.adjoint import DifferentiableRun
-
-    c = channel_case(fs, method, tmp_path)
-    s = make_solver(fs, method, c)
-    run = DifferentiableRun(s, T, body=c["body"])
-    inl = f32(c["inlet"]).requires_grad_()
-    *_, drag = run(*[f32(a) for a in c["u"]], rho0=f32(c["rho"]), inlet=inl)
-    loss = drag.mean()
-    loss.backward()
-
-    ref = torch_ref(method, c)
-    J = objective_b(ref, c)
-    x0 = d64(c["inlet"], (N, N)).requires_grad_()
-    Jx = J(x0)
-    Jx.backward()
-    assert abs(float(Jx)) > 1e-3
-    assert float(loss.detach()) == pytest.approx(float(Jx.detach()), rel=1e-4)
-    g_ad = inl.grad.reshape(N, N)
-    err = float((g_ad.double() - x0.grad).norm() / x0.grad.norm())
-    print(f"method {method}: drag {float(Jx):.5f}  rel L2 err vs FP64 autograd {err:.2e}")
-    assert err <= 1e-4
-    worst = check_fd(J, x0.detach(), g_ad)
-    print(f"method {method}: worst directional rel err vs central differences {worst:.2e}")
-    run.close()
-    s.close()
+def channel_flags(name):
+    fl = G["channel_flags"]
+    return fl | (G["channel_band"] * 2).astype(np.uint8) if name == "hybrid" else fl
 
 
-@pytest.mark.parametrize("method", [0, HOME])
-def test_segments_give_identical_gradients(fs, method, tmp_path):
-    """recomputation is bit-exact, so every checkpoint spacing gives the
-    same gradient bits; a second backward pass repeats the first"""
-    from myfluidsimulation_amd.adjoint import DifferentiableRun
+@pytest.mark.parametrize("name", METHODS)
+def test_tgv_inverse_gradient(fs, name):
+    from myfluidsimulation_amd.solver import Adjoint
 
-    c = channel_case(fs, method, tmp_path)
+    rho, u, _ = mg.tgv_inputs()
+    print(f"adjoint tgv {name}: N {N} T {T}", flush=True)
+    s = make_solver(fs, name, np.zeros(N ** 3, np.uint8))
+    adj = Adjoint(s, T)
+    try:
+        _, ux, uy, uz, _ = adj.forward(rho, *u)
+        tgt = G[f"tgv_{name}_target"]
+        du = [a - t for a, t in zip((ux, uy, uz), tgt)]
+        J = 0.5 * sum(float((d.astype(np.float64) ** 2).sum()) for d in du)
+        assert J == pytest.approx(float(G[f"tgv_{name}_J"]), rel=1e-4)
+        _, gx, gy, gz, _ = adj.backward(None, *du, None, want=(False, True, True, True, False))
+        grad = np.stack([gx, gy, gz])
+        err = rel_l2(grad, G[f"tgv_{name}_grad"])
+        print(f"{name}: J {J:.6e}  rel L2 err vs FP64 autograd {err:.2e}")
+        assert err <= 1e-4
+        worst = check_directions(grad, G[f"tgv_{name}_fd"], (3, N, N, N))
+        print(f"{name}: worst directional rel err vs central differences {worst:.2e}")
+    finally:
+        adj.close()
+        s.close()
+
+
+@pytest.mark.parametrize("name", METHODS)
+def test_channel_drag_gradient(fs, name):
+    from myfluidsimulation_amd.solver import Adjoint
+
+    print(f"adjoint channel {name}: N {N} T {T}", flush=True)
+    s = make_solver(fs, name, channel_flags(name))
+    adj = Adjoint(s, T, body=G["channel_body"])
+    try:
+        *_, drag = adj.forward(G["channel_rho"], G["channel_ux"], G["channel_uy"],
+                               G["channel_uz"], G["channel_inlet"])
+        J = float(drag.astype(np.float64).mean())
+        assert abs(J) > 1e-3
+        assert J == pytest.approx(float(G[f"channel_{name}_J"]), rel=1e-4)
+        gd = np.full(T, 1.0 / T, np.float32)
+        *_, gin = adj.backward(None, None, None, None, gd, want=(False, False, False, False, True))
+        err = rel_l2(gin, G[f"channel_{name}_grad"])
+        print(f"{name}: drag {J:.6f}  rel L2 err vs FP64 autograd {err:.2e}")
+        assert err <= 1e-4
+        worst = check_directions(gin, G[f"channel_{name}_fd"], (N, N))
+        print(f"{name}: worst directional rel err vs central differences {worst:.2e}")
+    finally:
+        adj.close()
+        s.close()
+
+
+@pytest.mark.parametrize("name", ["bgk", "home"])
+def test_segments_give_identical_gradients(fs, name):
+    """recomputation is bit-exact, so every checkpoint spacing gives the same
+    gradient bits, and a second backward pass repeats the first"""
+    from myfluidsimulation_amd.solver import Adjoint
+
     steps = 12
+    flags = channel_flags(name)
+    rng = np.random.default_rng(3)
+    gux = (1e-3 * rng.standard_normal(N ** 3)).astype(np.float32)
+    gd = np.linspace(0.5, 1.5, steps).astype(np.float32)
     out = []
     for seg in (1, 5, steps):
-        s = make_solver(fs, method, c)
-        run = DifferentiableRun(s, steps, segment=seg, body=c["body"])
-        assert run.info()["checkpoints"] == -(-steps // seg)
-        u0 = [f32(a).requires_grad_() for a in c["u"]]
-        inl = f32(c["inlet"]).requires_grad_()
-        _, ux, _, _, drag = run(*u0, rho0=f32(c["rho"]), inlet=inl)
-        loss = (ux * ux).sum() + drag.sum()
-        loss.backward(retain_graph=True)
-        first = [u0[0].grad.clone(), inl.grad.clone()]
-        loss.backward()
-        assert torch.equal(u0[0].grad, 2 * first[0])
-        assert torch.equal(inl.grad, 2 * first[1])
-        out.append(first)
-        run.close()
-        s.close()
+        print(f"adjoint segments {name}: segment {seg}", flush=True)
+        s = make_solver(fs, name, flags)
+        adj = Adjoint(s, steps, segment=seg, body=G["channel_body"])
+        try:
+            assert adj.info()["checkpoints"] == -(-steps // seg)
+            adj.forward(G["channel_rho"], G["channel_ux"], G["channel_uy"], G["channel_uz"],
+                        G["channel_inlet"])
+            first = adj.backward(None, gux, None, None, gd)
+            again = adj.backward(None, gux, None, None, gd)
+            for a, b in zip(first, again):
+                assert np.array_equal(a, b)
+            assert np.abs(first[1]).max() > 0 and np.abs(first[4]).max() > 0
+            out.append(first)
+        finally:
+            adj.close()
+            s.close()
     for o in out[1:]:
-        assert torch.equal(o[0], out[0][0]) and torch.equal(o[1], out[0][1])
+        for a, b in zip(o, out[0]):
+            assert np.array_equal(a, b)
 
 
 def test_adjoint_rejects_bad_use(fs):
-    from myfluidsimulation_amd.adjoint import Adjoint
     from myfluidsimulation_amd.capi import FS_ERR_STATE, FluidSimError
+    from myfluidsimulation_amd.solver import Adjoint
 
     s = fs.Solver(0, 16, 8, 8, 0.6)
     with pytest.raises(FluidSimError):
