@@ -165,3 +165,45 @@ def test_batch_random_domains(fs, seed):
             np.testing.assert_array_equal(b.state(i), st, err_msg=f"seed {seed} dom {i}")
             for q in range(4):
                 np.testing.assert_array_equal(b.macro(i, q), mac[q], err_msg=f"seed {seed} dom {i}")
+
+
+def test_seeded_sweep_64_domains(fs, oracle, tmp_path):
+    """The 64-domain batched sweep (seed 7: edges 32..128, tau 0.52..0.9,
+    mixed BGK / HOME / Hybrid with bands 1..3, half Taylor-Green, half
+    icosphere channels): after 101 batch steps (two-step launches and one
+    one-step launch) every domain equals its single-domain solver bit for
+    bit, and three domains (one per method) equal the FP32 oracle."""
+    steps = 101
+    specs = wl.batch_sweep_specs(64, 7)
+    built = [wl.build_sweep_domain(sp, str(tmp_path)) for sp in specs]
+    doms = []
+    for sp, (flags, fields, inlet) in zip(specs, built):
+        n = sp["nx"] * sp["ny"] * sp["nz"]
+        doms.append(dict(method=sp["method"], dims=(sp["nx"], sp["ny"], sp["nz"]), tau=sp["tau"],
+                         flags=np.zeros(n, np.uint8) if flags is None else flags, fields=fields,
+                         inlet=inlet, force=(0.0, 0.0, 0.0)))
+    with make_batch(fs, doms) as b:
+        b.step(steps)
+        for i, d in enumerate(doms):
+            print(f"sweep domain {i}: method {d['method']} dims {d['dims']}", flush=True)
+            mac, st = single(fs, d, steps)
+            np.testing.assert_array_equal(b.state(i), st, err_msg=f"dom {i} state")
+            for k in range(4):
+                np.testing.assert_array_equal(b.macro(i, k), mac[k], err_msg=f"dom {i} macro {k}")
+        picked = {}
+        for i, d in enumerate(doms):  # the smallest domain of each method
+            n = int(np.prod(d["dims"]))
+            if d["method"] not in picked or n < picked[d["method"]][1]:
+                picked[d["method"]] = (i, n)
+        assert len(picked) == 3
+        for i, _ in picked.values():
+            d = doms[i]
+            o = oracle.Oracle(d["method"], *d["dims"], d["tau"])
+            o.set_flags(d["flags"])
+            if d["inlet"] is not None:
+                o.set_inlet(d["inlet"])
+            o.reinit(*d["fields"])
+            o.step(steps)
+            np.testing.assert_array_equal(b.state(i), o.state(), err_msg=f"dom {i} vs oracle")
+            for k in range(4):
+                np.testing.assert_array_equal(b.macro(i, k), o.macro(k), err_msg=f"dom {i} macro")

@@ -10,7 +10,9 @@
   reverse under y -> -y, both even under z -> -z; the w the nonlinear term
   generates odd under z -> -z, even under x and y) hold.
 * The 768x256x256 channel's icosphere (R=40): the voxelized flag grid equals
-  the oracle voxelizer's and its band (d0 = 1, 2, 4) the oracle band.
+  the oracle voxelizer's and its band (d0 = 1, 2, 4) the oracle band; BGK,
+  HOME and Hybrid with bands 1, 2, 4 on that channel equal the FP32 oracle bit
+  for bit after 11 steps (fields and full state).
 """
 import os
 
@@ -111,3 +113,43 @@ def test_channel768_geometry_exact(fs, oracle, tmp_path):
     solid = (flags & 1).astype(np.uint8)
     for d0 in (1, 2, 4):
         assert np.array_equal(fs.band(solid, nx, ny, nz, d0), oracle.band(solid, nx, ny, nz, d0))
+
+
+@pytest.fixture(scope="module")
+def channel768_flags(fs, tmp_path_factory):
+    nx, ny, nz = 768, 256, 256
+    tris, bmin, bmax, scale, trans = channel(fs, str(tmp_path_factory.mktemp("c768")), nx, ny, nz,
+                                             (192, 128, 128), 40.0)
+    return wl.channel_flags(nx, ny, nz, fs.voxelize(tris, bmin, bmax, nx, ny, nz, scale, trans))
+
+
+@pytest.mark.parametrize("method,band", [(0, 0), (1, 0), (2, 1), (2, 2), (2, 4)],
+                         ids=["bgk", "home", "hybrid_band1", "hybrid_band2", "hybrid_band4"])
+def test_channel768_solvers_exact(fs, oracle, channel768_flags, method, band):
+    """The 768x256x256 roofline channel (tau 0.56, inlet u 0.05, outlet,
+    walls, icosphere R=40): 11 steps (five two-step launches and one
+    one-step launch for BGK / Hybrid) equal the FP32 oracle bit for bit,
+    fields and full state."""
+    nx, ny, nz, tau, u_in, steps = 768, 256, 256, 0.56, 0.05, 11
+    print(f"channel768 method {method} band {band}", flush=True)
+    flags = channel768_flags
+    if band:
+        flags = flags | (fs.band((flags & 1).astype(np.uint8), nx, ny, nz, band) * 2).astype(
+            np.uint8)
+    fields = wl.uniform_flow(nx * ny * nz, u_in)
+    inlet = np.full(ny * nz, u_in, np.float32)
+    s = fs.Solver(method, nx, ny, nz, tau)
+    s.set_flags(flags)
+    s.set_inlet_profile(inlet)
+    s.reinit_from_macro(*fields)
+    s.step(steps)
+    o = oracle.Oracle(method, nx, ny, nz, tau)
+    o.set_flags(flags)
+    o.set_inlet(inlet)
+    o.reinit(*fields)
+    o.step(steps)
+    for k in range(4):
+        assert np.array_equal(s.macro(k), o.macro(k)), f"macro {k}"
+    assert np.array_equal(s.state(), o.state())
+    s.close()
+    o.close()

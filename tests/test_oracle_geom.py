@@ -211,3 +211,86 @@ def test_vtk_writer_format(oracle, tmp_path):
     q = os.path.join(tmp_path, "v.vtk")
     oracle.write_vector_vtk(q, field[:2], field[2:4], field[4:6], 2, 1, 1)
     assert open(q).read().endswith("VECTORS vec float\n1 1.23457 1e+20\n0.5 -2.5e-07 0\n")
+
+
+# ---------------------------------------------------------------- field helpers
+# kern_speed (field_ops.cu:12-18), kern_rho_slab / kern_rho_gauss
+# (init_profiles.cu:7-29): pinned by closed forms and exact special cases
+# chosen so that a dropped component, a corner-instead-of-centre coordinate
+# or a sigma / 2 sigma^2 slip changes the result.
+
+def test_speed_exact_quadruples_and_float64(oracle):
+    # integer Pythagorean quadruples: every product, sum and the root are
+    # exact in float32, whatever the component order or signs
+    quads = [(2, 3, 6, 7), (1, 4, 8, 9), (2, 6, 9, 11), (1, 2, 2, 3), (4, 4, 7, 9),
+             (2, 10, 11, 15), (0, 0, 5, 5), (0, 3, 0, 3)]
+    u = []
+    want = []
+    for a, b, c, d in quads:
+        for perm in ((a, b, c), (c, a, b), (b, c, a)):
+            for sgn in ((1, 1, 1), (-1, 1, -1), (1, -1, -1)):
+                for scale in (1.0, 2.0 ** -7, 2.0 ** 5):
+                    u.append([s * p * scale for s, p in zip(sgn, perm)])
+                    want.append(d * scale)
+    u = np.array(u, np.float32)
+    got = oracle.speed(u[:, 0], u[:, 1], u[:, 2])
+    np.testing.assert_array_equal(got, np.array(want, np.float32))
+    # random vectors: within 2 ulp of the float64 magnitude of the float32
+    # inputs (products, two sums and the root each round once)
+    rng = np.random.default_rng(17)
+    v = (rng.standard_normal((20000, 3)) * 10.0 ** rng.uniform(-4, 1, (20000, 1))).astype(np.float32)
+    got = oracle.speed(v[:, 0], v[:, 1], v[:, 2]).astype(np.float64)
+    ref = np.sqrt((v.astype(np.float64) ** 2).sum(axis=1))
+    ulp = np.spacing(ref.astype(np.float32)).astype(np.float64)
+    assert np.all(np.abs(got - ref) <= 2.0 * ulp)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_rho_slab_cell_centres(oracle, axis):
+    """The slab holds the cells whose centre s = (i + 0.5)/N satisfies
+    |s - centre| < width/2: with N = 20, centre 0.5, width 0.5 that is
+    i = 5..14 (a corner coordinate i/N would give i = 6..14)."""
+    dims = [24, 18, 14]
+    dims[axis] = 20
+    nx, ny, nz = dims
+    amp = np.float32(3e-3)
+    rho = oracle.rho_slab(nx, ny, nz, axis, amp, 0.5, 0.5).reshape(nz, ny, nx)
+    idx = np.indices((nz, ny, nx))[2 - axis]
+    inside = (idx >= 5) & (idx <= 14)
+    assert np.array_equal(rho[inside], np.full(inside.sum(), np.float32(1) + amp))
+    assert np.all(rho[~inside] == np.float32(1.0))
+    # an off-centre slab, counted from the cell centres in float64
+    rho = oracle.rho_slab(nx, ny, nz, axis, amp, 0.3125, 0.2).reshape(nz, ny, nx)
+    line = np.moveaxis(rho, 2 - axis, 0)[:, 3, 2]
+    n = dims[axis]
+    s = (np.arange(n) + 0.5) / n
+    cells = np.flatnonzero(np.abs(s - 0.3125) < 0.1)
+    assert np.array_equal(np.flatnonzero(line != np.float32(1.0)), cells)
+    assert np.all(np.moveaxis(rho, 2 - axis, 0) == line[:, None, None])
+
+
+def test_rho_gauss_peak_sigma_and_closed_form(oracle):
+    """Cell centres of a 16-cell axis are exact binary fractions: centred on
+    cell 8 (0.53125), the peak is 1 + amp exactly, and with sigma = 1/16 the
+    next cell along x sits at exactly one sigma, where the profile is
+    1 + amp e^{-1/2} (not e^{-1}, as a missing factor 2 would give)."""
+    n = 16
+    amp, sig = np.float32(0.01), np.float32(1.0 / 16.0)
+    c = 8.5 / 16
+    rho = oracle.rho_gauss(n, n, n, c, c, c, sig, amp).reshape(n, n, n)
+    assert rho[8, 8, 8] == np.float32(1.0) + amp
+    one_sigma = 1.0 + float(amp) * np.exp(-0.5)
+    for cell in ((8, 8, 9), (8, 9, 8), (9, 8, 8), (8, 8, 7)):
+        assert abs(float(rho[cell]) - one_sigma) <= 2 * np.spacing(np.float32(1.0))
+    assert abs(float(rho[8, 9, 9]) - (1.0 + float(amp) * np.exp(-1.0))) <= 2 * np.spacing(
+        np.float32(1.0))
+    # everywhere: the float64 closed form at the float32 inputs, within a few ulp of 1
+    nx, ny, nz = 30, 22, 12
+    cx, cy, cz = np.float32(0.41), np.float32(0.6), np.float32(0.37)
+    sig, amp = np.float32(0.15), np.float32(0.02)
+    rho = oracle.rho_gauss(nx, ny, nz, cx, cy, cz, sig, amp).reshape(nz, ny, nx).astype(np.float64)
+    z, y, x = np.meshgrid((np.arange(nz) + 0.5) / nz, (np.arange(ny) + 0.5) / ny,
+                          (np.arange(nx) + 0.5) / nx, indexing="ij")
+    r2 = (x - float(cx)) ** 2 + (y - float(cy)) ** 2 + (z - float(cz)) ** 2
+    ref = 1.0 + float(amp) * np.exp(-r2 / (2.0 * float(sig) ** 2))
+    assert np.max(np.abs(rho - ref)) <= 4 * np.spacing(np.float32(1.0))
