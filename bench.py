@@ -1,25 +1,26 @@
 """Benchmark: D3Q19 lattice updates per second on one MI355X.
 
-Default workload (the configuration the headline metric is quoted on):
-obstacle channel 384x128x128, FP32, tau 0.56, inlet u=0.05 at x=0
-(equilibrium), zero-gradient outlet at x=383, solid y/z wall planes, and a
-seeded jittered icosphere (3 subdivisions, 1280 faces, radius 20 cells,
-centre (96,64,64)) written as binary STL and voxelized on the GPU.  BGK (the
-headline `value`), HOME and Hybrid(B0) band 2 each run W untimed then K
-timed steps; one "step" is one lattice time step over the whole domain.
+Default workload ("roofline"): the large single-domain roofline runs, FP32.
+  value    BGK on the 512^3 periodic Taylor-Green vortex (tau 0.6, u0 0.05,
+           equilibrium start), HOME beside it (`methods`);
+  channel768  the 768x256x256 obstacle channel (tau 0.56, inlet u 0.05 at
+           x=0, zero-gradient outlet, solid y/z walls, a seeded jittered
+           icosphere of radius 40 cells written as binary STL and voxelized
+           on the GPU) with BGK, HOME and Hybrid(B0) bands 1, 2 and 4;
+  extras   the 384x128x128 obstacle channel (radius 20; Hybrid band 2), a
+           shorter run of the 64-domain batched sweep and the geometry path
+           (voxelize / band ms per call) -- skipped with --no-extras;
+  cpu_baseline  the FP32 CPU oracle on the 512^3 vortex (and the 768 channel
+           in cpu_baseline_channel768): 5 untimed + 20 timed steps.
+Each method runs W untimed then K timed steps (defaults 50 and 200); one
+"step" is one lattice time step over the whole domain.
 
-Other workloads (--workload), the large single-domain roofline runs:
-  tgv256      256^3 periodic Taylor-Green vortex, BGK and HOME
-  tgv512      512^3 periodic Taylor-Green vortex, BGK and HOME
-  channel768  768x256x256 channel, icosphere radius 40, Hybrid bands 1, 2, 4
-              (and BGK, HOME)
+Other workloads (--workload): tgv512, channel768, channel384, tgv256 alone;
   batch64     the 64-domain batched sweep against the same runs one by one
   adjoint128  differentiable 128^3 Taylor-Green run (T = 2000, sqrt(T)
               checkpoints): forward + backward MLUPS against plain steps
   voxel       the geometry path: STL parse, voxelization (and AMR 2), band,
               wall ms per C-ABI call against the single-threaded oracle
-The default channel line also carries a shorter run of the batched sweep and
-the geometry path of its obstacle (`geometry`: voxelize / band ms per call).
 
 Timing: HIP events recorded on the solver's stream around the K steps
 (fs_solver_timed_steps).  HOME and the one-step BGK / Hybrid kernel launch
@@ -35,6 +36,11 @@ two per cell for the two-step kernel); peak = 8.0 TB/s HBM3E.  The two-step
 kernel itself moves 76 B per update (kernel_B_per_LUP, kernel_GBps), and
 `traffic` / measured_traffic_GBps give what the counters saw.  MLUPS counts
 every cell, solid included.
+
+`traffic` (HBM bytes per launch) comes from the committed rocprofv3
+FETCH_SIZE / WRITE_SIZE summaries, looked up by workload, kernel and kernel
+variant (profiles/traffic.json); it is null when no profile measured the
+variant that ran.
 
 cpu_baseline: the FP32 CPU oracle (scalar C++ restatement of the reference
 rule, OpenMP over z-planes, all host threads) timed on a bounded sample of
@@ -154,8 +160,9 @@ def copy_ceiling():
     return 2.0 * nbytes * 10 / (ms.value * 1e-3) / 1e9
 
 
-def cpu_baseline(case, budget_s=12.0):
-    """FP32 oracle (BGK) on the same workload, OpenMP on all host threads."""
+def cpu_baseline(case, budget_s=12.0, warmup_steps=1, timed_steps=None):
+    """FP32 oracle (BGK) on the same workload, OpenMP on all host threads:
+    warmup_steps untimed, then timed_steps (or as many as fit budget_s)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle  # the checker, used here only as the timed CPU baseline
 
@@ -166,17 +173,29 @@ def cpu_baseline(case, budget_s=12.0):
         o.set_inlet(case.inlet)
     o.reinit(*case.fields)
     t0 = time.perf_counter()
-    o.step(1)
-    one = time.perf_counter() - t0
-    k = int(max(2, min(200, budget_s / max(one, 1e-3))))
+    o.step(warmup_steps)
+    one = (time.perf_counter() - t0) / max(1, warmup_steps)
+    k = timed_steps or int(max(2, min(200, budget_s / max(one, 1e-3))))
     t0 = time.perf_counter()
     o.step(k)
     dt = time.perf_counter() - t0
     o.close()
+    import platform
+
+    cpu = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    cpu = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {"value": round(case.n * k / dt / 1e6, 2), "unit": "MLUPS",
             "cores": pyoracle.num_threads(), "kind": "port",
-            "sample": f"{k} BGK steps of the {case.nx}x{case.ny}x{case.nz} workload "
-                      f"(FP32 oracle, OpenMP, {dt:.1f} s)"}
+            "sample": f"{warmup_steps} untimed + {k} timed BGK steps of the "
+                      f"{case.nx}x{case.ny}x{case.nz} workload (FP32 oracle, OpenMP, "
+                      f"{dt:.1f} s timed, {cpu})"}
 
 
 def batch_bench(steps, warmup, seed=7):
@@ -379,17 +398,89 @@ def voxel_bench(reps, cpu=True, labels=None):
     return line
 
 
+def traffic_lookup(workload, method, kernel, variant):
+    """HBM bytes per launch of (workload, method, kernel, variant) from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summaries
+    (profiles/traffic.json, counters calibrated against the copy kernel), or
+    None when no profile measured exactly that kernel variant there."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        entries = json.load(fh)["entries"]
+    for e in entries:
+        if (e["workload"], e["method"], e["kernel"], e["variant"]) == (workload, method, kernel,
+                                                                       variant):
+            return e["bytes_per_launch"]
+    return None
+
+
+def roofline_of(r, traffic, ceiling):
+    return {"bound": "hbm", "achieved": r["achieved_GBps"], "peak": PEAK_HBM_GBPS,
+            "unit": "GB/s", "frac": r["roofline_frac"], "traffic": traffic,
+            "algorithmic_bytes_per_launch": r["bytes_per_launch"],
+            "launch_ms": r["launch_ms"], "kernel": r["kernel"], "variant": r["variant"],
+            "kernel_B_per_LUP": r["kernel_B_per_LUP"], "kernel_GBps": r["kernel_GBps"],
+            "measured_traffic_GBps": (round(traffic / (r["launch_ms"] * 1e-3) / 1e9, 1)
+                                      if traffic else None),
+            "measured_copy_ceiling_GBps": round(ceiling, 1)}
+
+
+def run_set(case, runs, steps, warmup, workload, tag=""):
+    """runs: (name, method, band) -> {name: result}, each result carrying the
+    profiled traffic of its kernel variant on this workload (or None)"""
+    out = {}
+    for name, m, band in runs:
+        r = run_method(case, m, steps, warmup, band)
+        r["traffic"] = traffic_lookup(workload, NAMES[m], r["kernel"], r["variant"])
+        out[name] = r
+        print(f"# {tag}{name}: {r['MLUPS']:.1f} MLUPS, {r['ms_per_step'] * 1e3:.1f} us/step, "
+              f"{r['achieved_GBps']:.0f} GB/s ({100 * r['roofline_frac']:.1f}%)",
+              file=sys.stderr, flush=True)
+    return out
+
+
+WORKLOADS = {
+    # name: (case builder, runs, default steps, default warm-up, description)
+    "tgv512": (lambda: tgv_case(512), [("bgk", 0, 0), ("home", 1, 0)], 200, 50,
+               "Taylor-Green vortex 512^3 periodic, tau 0.6, u0 0.05"),
+    "channel768": (lambda: channel_case(768, 256, 256, (192, 128, 128), 40.0),
+                   [("bgk", 0, 0), ("home", 1, 0), ("hybrid_band1", 2, 1),
+                    ("hybrid_band2", 2, 2), ("hybrid_band4", 2, 4)], 200, 50,
+                   "obstacle channel 768x256x256, tau 0.56, inlet u 0.05, zero-gradient outlet, "
+                   f"y/z walls, icosphere R40 seed {SEED}"),
+    "channel384": (lambda: channel_case(384, 128, 128, (96, 64, 64), 20.0),
+                   [("bgk", 0, 0), ("home", 1, 0), ("hybrid", 2, 2)], 5000, 100,
+                   "obstacle channel 384x128x128, tau 0.56, inlet u 0.05, zero-gradient outlet, "
+                   f"y/z walls, icosphere R20 seed {SEED}; Hybrid band 2"),
+    "tgv256": (lambda: tgv_case(256), [("bgk", 0, 0), ("home", 1, 0)], 400, 50,
+               "Taylor-Green vortex 256^3 periodic, tau 0.6, u0 0.05"),
+}
+
+
+def base_line(value, steps, warmup, ms_per_step, workload, extra_config=None):
+    cfg = {"workload": workload}
+    cfg.update(extra_config or {})
+    return {
+        "metric": "MLUPS & % HBM roofline (BGK/HOME/hybrid); batched & adjoint MLUPS vs plain",
+        "value": value, "unit": "MLUPS", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded)", "config": cfg,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="channel",
-                    choices=["channel", "tgv256", "tgv512", "channel768", "batch64",
-                             "adjoint128", "voxel"])
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-batch", action="store_true",
-                    help="skip the 64-domain batched sweep summary of the channel workload")
+    ap.add_argument("--workload", default="roofline",
+                    choices=["roofline", "tgv512", "channel768", "channel384", "tgv256",
+                             "batch64", "adjoint128", "voxel"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="roofline workload: only the 512^3 and 768x256x256 runs (no 384x128x128 "
+                         "channel, batched sweep or geometry summaries)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -423,85 +514,63 @@ def main():
         print(json.dumps(batch_bench(args.steps or 2000, 20 if args.warmup is None
                                      else args.warmup)))
         return
-    if args.workload == "channel":
-        case = channel_case(384, 128, 128, (96, 64, 64), 20.0)
-        runs = [("bgk", 0, 0), ("home", 1, 0), ("hybrid", 2, 2)]
-        steps, warmup = args.steps or 5000, 100 if args.warmup is None else args.warmup
-        desc = ("obstacle channel 384x128x128, tau 0.56, inlet u 0.05, zero-gradient outlet, "
-                f"y/z walls, icosphere R20 seed {SEED}; value = BGK, also HOME and Hybrid band 2")
-    elif args.workload == "tgv256":
-        case = tgv_case(256)
-        runs = [("bgk", 0, 0), ("home", 1, 0)]
-        steps, warmup = args.steps or 400, 50 if args.warmup is None else args.warmup
-        desc = "Taylor-Green vortex 256^3 periodic, tau 0.6, u0 0.05; value = BGK, also HOME"
-    elif args.workload == "tgv512":
-        case = tgv_case(512)
-        runs = [("bgk", 0, 0), ("home", 1, 0)]
-        steps, warmup = args.steps or 200, 50 if args.warmup is None else args.warmup
-        desc = "Taylor-Green vortex 512^3 periodic, tau 0.6, u0 0.05; value = BGK, also HOME"
-    else:
-        case = channel_case(768, 256, 256, (192, 128, 128), 40.0)
-        runs = [("bgk", 0, 0), ("home", 1, 0), ("hybrid_band1", 2, 1),
-                ("hybrid_band2", 2, 2), ("hybrid_band4", 2, 4)]
-        steps, warmup = args.steps or 200, 50 if args.warmup is None else args.warmup
-        desc = ("obstacle channel 768x256x256, tau 0.56, icosphere R40; value = BGK, also HOME "
-                "and Hybrid bands 1, 2, 4")
 
-    methods = {}
-    for name, m, band in runs:
-        methods[name] = run_method(case, m, steps, warmup, band)
-        r = methods[name]
-        print(f"# {name}: {r['MLUPS']:.1f} MLUPS, {r['ms_per_step'] * 1e3:.1f} us/step, "
-              f"{r['achieved_GBps']:.0f} GB/s", file=sys.stderr, flush=True)
+    if args.workload != "roofline":
+        build, runs, dsteps, dwarm, desc = WORKLOADS[args.workload]
+        steps = args.steps or dsteps
+        warmup = dwarm if args.warmup is None else args.warmup
+        case = build()
+        methods = run_set(case, runs, steps, warmup, args.workload)
+        head = methods[runs[0][0]]
+        line = base_line(head["MLUPS"], steps, warmup, head["ms_per_step"],
+                         desc + f"; value = {runs[0][0]}",
+                         {"cells": case.n, "fluid_cells": case.fluid()})
+        line["methods"] = methods
+        line["roofline"] = roofline_of(head, head["traffic"], copy_ceiling())
+        if not args.no_cpu:
+            line["cpu_baseline"] = cpu_baseline(case)
+        print(json.dumps(line))
+        return
+
+    # The large single-domain roofline runs: value = BGK on the 512^3
+    # Taylor-Green vortex, HOME beside it; the 768x256x256 channel with BGK,
+    # HOME and Hybrid bands 1, 2, 4; then the 384x128x128 channel, the
+    # batched sweep and the geometry path as extra keys.
+    steps = args.steps or 200
+    warmup = 50 if args.warmup is None else args.warmup
+    build, runs, _, _, desc512 = WORKLOADS["tgv512"]
+    case512 = build()
+    methods = run_set(case512, runs, steps, warmup, "tgv512", "tgv512 ")
+    head = methods["bgk"]
     ceiling = copy_ceiling()
-    head = methods[runs[0][0]]
-    # HBM bytes per BGK launch from the committed rocprofv3 FETCH_SIZE /
-    # WRITE_SIZE runs of this workload (calibrated against the copy kernel)
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_channel384.json")
-    if args.workload == "channel" and os.path.exists(tpath):
-        with open(tpath) as fh:
-            prof = json.load(fh)
-        # only when the profile measured the kernel that ran here
-        if prof.get("kernel", "k_pop_step") == head["kernel"]:
-            traffic = prof.get("bgk_bytes_per_launch")
-    line = {
-        "metric": "MLUPS & % HBM roofline (BGK/HOME/hybrid); batched & adjoint MLUPS vs plain",
-        "value": head["MLUPS"],
-        "unit": "MLUPS",
-        "n_gpus": 1,
-        "steps": steps,
-        "warmup": warmup,
-        "ms_per_step": head["ms_per_step"],
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded)",
-        "config": {"workload": desc, "cells": case.n, "fluid_cells": case.fluid()},
-        "methods": methods,
-        "roofline": {"bound": "hbm", "achieved": head["achieved_GBps"], "peak": PEAK_HBM_GBPS,
-                     "unit": "GB/s", "frac": head["roofline_frac"], "traffic": traffic,
-                     "algorithmic_bytes_per_launch": head["bytes_per_launch"],
-                     "launch_ms": head["launch_ms"], "kernel": head["kernel"],
-                     "kernel_B_per_LUP": head["kernel_B_per_LUP"],
-                     "kernel_GBps": head["kernel_GBps"],
-                     "measured_traffic_GBps": (round(traffic / (head["launch_ms"] * 1e-3) / 1e9, 1)
-                                               if traffic else None),
-                     "measured_copy_ceiling_GBps": round(ceiling, 1)},
-    }
-    if args.workload == "channel" and not args.no_batch:
+    build, runs, _, _, desc768 = WORKLOADS["channel768"]
+    case768 = build()
+    ch768 = run_set(case768, runs, steps, warmup, "channel768", "channel768 ")
+    line = base_line(head["MLUPS"], steps, warmup, head["ms_per_step"],
+                     f"{desc512}: value = BGK, HOME beside it; channel768 = {desc768}: BGK, "
+                     "HOME, Hybrid bands 1, 2, 4",
+                     {"cells": case512.n, "fluid_cells": case512.fluid(),
+                      "channel768_cells": case768.n, "channel768_fluid_cells": case768.fluid()})
+    line["methods"] = methods
+    line["channel768"] = ch768
+    line["roofline"] = roofline_of(head, head["traffic"], ceiling)
+    line["roofline_home"] = roofline_of(methods["home"], methods["home"]["traffic"], ceiling)
+    if not args.no_extras:
+        build, runs, _, _, desc384 = WORKLOADS["channel384"]
+        case384 = build()
+        line["channel384"] = run_set(case384, runs, steps, warmup, "channel384", "channel384 ")
+        line["channel384"]["workload"] = desc384
         b = batch_bench(300, 20)
         line["batched"] = {k: b[k] for k in ("value", "unit", "ms_per_step",
                                              "sequential_sync_step1_MLUPS",
                                              "sequential_async_MLUPS", "speedup_vs_sync_step1",
                                              "speedup_vs_async", "finite")}
         line["batched"]["workload"] = b["config"]["workload"]
-    if args.workload == "channel":
         line["geometry"] = voxel_bench(3, cpu=not args.no_cpu,
-                                       labels=("channel384", "channel384_amr2"))["cases"]
+                                       labels=("channel384", "channel768"))["cases"]
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(case)
+        line["cpu_baseline"] = cpu_baseline(case512, warmup_steps=5, timed_steps=20)
+        line["cpu_baseline_channel768"] = cpu_baseline(case768, warmup_steps=5, timed_steps=20)
     print(json.dumps(line))
 
 
