@@ -588,11 +588,13 @@ int fs_solver_autotune(fs_solver* s, int trial_steps, int* chosen) {
     return FS_OK;
   }
   // time every applicable variant from the current state, restoring it after
-  // each trial, and keep the fastest; population trials run an even number
-  // of steps from the natural representation
+  // each trial (populations and the macro planes they reported), and keep the
+  // fastest; population trials run an even number of steps from the natural
+  // representation
   const size_t bytes = sizeof(float) * (home ? 10 : 19) * s->g.P;
+  const size_t macroBytes = home ? 0 : sizeof(float) * 4 * s->g.P;
   float* saved = nullptr;
-  FS_HIP(hipMalloc((void**)&saved, bytes));
+  FS_HIP(hipMalloc((void**)&saved, bytes + macroBytes));
   const int cur0 = s->cur, parity0 = s->parity;
   float* const F0 = s->F;
   float* const F20 = s->F2;
@@ -600,8 +602,11 @@ int fs_solver_autotune(fs_solver* s, int trial_steps, int* chosen) {
   int best = -1;
   float best_ms = 0.0f;
   int rc = FS_OK;
+  float* const savedMacro = saved + bytes / sizeof(float);
   hipError_t e = hipMemcpyAsync(saved, home ? s->mbuf[cur0] : s->F, bytes,
                                 hipMemcpyDeviceToDevice, s->stream);
+  if (e == hipSuccess && !home)
+    e = hipMemcpyAsync(savedMacro, s->macro, macroBytes, hipMemcpyDeviceToDevice, s->stream);
   auto restore = [&]() {
     if (home) {
       s->cur = cur0;
@@ -610,7 +615,11 @@ int fs_solver_autotune(fs_solver* s, int trial_steps, int* chosen) {
     s->F = F0;
     s->F2 = F20;
     s->parity = parity0;
-    return hipMemcpyAsync(s->F, saved, bytes, hipMemcpyDeviceToDevice, s->stream);
+    // F2's solid slots still equal F's (no kernel writes a solid slot)
+    hipError_t r = hipMemcpyAsync(s->F, saved, bytes, hipMemcpyDeviceToDevice, s->stream);
+    if (r == hipSuccess)
+      r = hipMemcpyAsync(s->macro, savedMacro, macroBytes, hipMemcpyDeviceToDevice, s->stream);
+    return r;
   };
   for (int v = 0; v < nvar && e == hipSuccess && rc == FS_OK; ++v) {
     if (home && !fs::home_variant_ok(s->g, v)) continue;

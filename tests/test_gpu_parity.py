@@ -299,10 +299,12 @@ def test_pop_variants_and_autotune(fs, oracle, method):
             s.set_variant(variant)
             s.reinit_from_macro(*fields)
             if variant == -1:
-                before = s.state()
+                before, mac_before = s.state(), s.macros()
                 chosen = s.autotune(3)
                 assert chosen in (0, 1, 2, 3, 4, 5) and s.variant == chosen
                 np.testing.assert_array_equal(s.state(), before)
+                for g, r in zip(s.macros(), mac_before):  # the macro planes too
+                    np.testing.assert_array_equal(g, r)
             else:
                 assert s.variant == variant
             for k, (st, mac) in zip(chunks, refs):
