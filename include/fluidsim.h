@@ -287,6 +287,18 @@ int fs_voxelize(const float* host_tri_xyz, int ntri, const float* bbmin3,
  * the GPU: band = fluid cells within 6-neighbour distance d0 of a solid. */
 int fs_band(const uint8_t* host_solid, int nx, int ny, int nz, int d0,
             uint8_t* host_band_out);
+/* Extension with no counterpart in the reference, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): fs_band on caller-owned device buffers
+ * (nx*ny*nz bytes each), enqueued on hip_stream (NULL: the null stream) and
+ * returning without waiting.  It calls no hipMalloc / hipFree: the BFS
+ * distances live in a fixed 64 MiB device array inside the library (one per
+ * GPU, held until the process ends), so grids needing more scratch -- more
+ * than 64 Mi cells, or 16 Mi cells when d0 >= 255 needs int distances -- are
+ * rejected with FS_ERR_INVALID.  Calls that overlap on one GPU share that
+ * array and race: serialise them. */
+int fs_band_noalloc(const uint8_t* d_solid, int nx, int ny, int nz, int d0,
+                    uint8_t* d_band, void* hip_stream);
 
 /* ===================== fields and output ================================ */
 

@@ -96,6 +96,7 @@ SIGNATURES = {
     "fs_load_stl": (_i, [C.c_char_p, _p, _i, C.POINTER(_i), _p, _p]),
     "fs_voxelize": (_i, [_p, _i, _p, _p, _i, _i, _i, _f, _f, _f, _f, _i, _f, _p]),
     "fs_band": (_i, [_p, _i, _i, _i, _i, _p]),
+    "fs_band_noalloc": (_i, [_p, _i, _i, _i, _i, _p, _p]),
     "fs_speed": (_i, [_p, _p, _p, _p, _sz]),
     "fs_diff": (_i, [_p, _p, _p, _sz]),
     "fs_shift": (_i, [_p, _f, _p, _sz]),

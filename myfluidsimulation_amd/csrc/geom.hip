@@ -14,6 +14,9 @@
 //   3. solid = surface or not exterior; optional AMR coverage downsample.
 // Band (sim_runner.hpp:28-91): a level-synchronous BFS, one launch per
 // distance ring, non-periodic, then band = fluid cells with 1 <= dist <= d0.
+#include <mutex>
+
+#include "fluidsim.h"
 #include "fs_internal.hpp"
 
 namespace fs {
@@ -459,4 +462,65 @@ hipError_t band_device(const uint8_t* d_solid, int nx, int ny, int nz, int d0, u
   return band_run<int>(d_solid, nx, ny, nz, d0, d_band, reinterpret_cast<int*>(d_scratch), st);
 }
 
+// ---------------------------------------------------------------------------
+// Band without allocation (fs_band_noalloc): the distance scratch of the BFS
+// lives in a file-scope device array of kBandScratchBytes, one copy per GPU,
+// held until the process ends.  Host code never passes the array's name to
+// a kernel (in host code it is only a stand-in for the symbol): the device
+// address of each GPU's copy is looked up once per device with
+// hipGetSymbolAddress and kept.  k_dist_init writes every distance a call
+// reads, so nothing has to start at zero; calls that overlap on one GPU
+// share the array and race.
+constexpr size_t kBandScratchBytes = size_t(64) << 20;
+__device__ __align__(256) uint8_t scratch[kBandScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+size_t band_noalloc_max_bytes() { return kBandScratchBytes; }
+
+hipError_t band_scratch_address(void** out) {
+  *out = nullptr;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
+
 }  // namespace fs
+
+extern "C" int fs_band_noalloc(const uint8_t* d_solid, int nx, int ny, int nz, int d0,
+                               uint8_t* d_band, void* hip_stream) {
+  if (!d_solid || !d_band || nx < 1 || ny < 1 || nz < 1)
+    return fs::fail(FS_ERR_INVALID, "bad band arguments");
+  const size_t n = size_t(nx) * ny * nz;
+  if (n >= (size_t(1) << 31)) return fs::fail(FS_ERR_INVALID, "grid of %zu cells exceeds 2^31", n);
+  const int dd = d0 < 0 ? 0 : d0;
+  const size_t need = fs::band_scratch_bytes(nx, ny, nz, dd);
+  if (need > fs::kBandScratchBytes)
+    return fs::fail(FS_ERR_INVALID,
+                    "fs_band_noalloc: %dx%dx%d cells with d0 = %d need %zu scratch bytes, more "
+                    "than its fixed %zu (use fs_band)",
+                    nx, ny, nz, d0, need, fs::kBandScratchBytes);
+  if (int rc = fs::runtime_check()) return rc;
+  void* scr = nullptr;
+  hipError_t e = fs::band_scratch_address(&scr);
+  if (e != hipSuccess)
+    return fs::fail(FS_ERR_DEVICE, "fs_band_noalloc: scratch address: %s", hipGetErrorString(e));
+  e = fs::band_device(d_solid, nx, ny, nz, dd, d_band, static_cast<uint8_t*>(scr),
+                      static_cast<hipStream_t>(hip_stream));
+  if (e != hipSuccess) return fs::fail(FS_ERR_DEVICE, "fs_band_noalloc: %s", hipGetErrorString(e));
+  return FS_OK;
+}
