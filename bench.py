@@ -477,6 +477,9 @@ def main():
     ap.add_argument("--workload", default="roofline",
                     choices=["roofline", "tgv512", "channel768", "channel384", "tgv256",
                              "batch64", "adjoint128", "voxel"])
+    ap.add_argument("--methods", default=None,
+                    help="single-workload runs: comma-separated subset of the workload's "
+                         "methods (e.g. home), for profiling one kernel")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-extras", action="store_true",
                     help="roofline workload: only the 512^3 and 768x256x256 runs (no 384x128x128 "
@@ -517,6 +520,12 @@ def main():
 
     if args.workload != "roofline":
         build, runs, dsteps, dwarm, desc = WORKLOADS[args.workload]
+        if args.methods:
+            keep = args.methods.split(",")
+            runs = [r for r in runs if r[0] in keep]
+            if not runs:
+                print(f"no method of {args.workload} in {args.methods}", file=sys.stderr)
+                sys.exit(2)
         steps = args.steps or dsteps
         warmup = dwarm if args.warmup is None else args.warmup
         case = build()

@@ -310,3 +310,22 @@ def test_home_reference_table_option(oracle):
     pop = mk(oracle, 0, (4, 4, 4), 0.7)
     with pytest.raises(RuntimeError):
         pop.set_bounce(1)
+
+
+@pytest.mark.parametrize("dims", [(64, 64, 64), (130, 33, 17), (37, 41, 43), (256, 8, 96)])
+def test_taylor_green_fields_closed_form(dims):
+    """workloads.taylor_green_3d (per-axis factors, broadcast) equals the
+    full-grid evaluation of u = u0 sin kx cos ky cos kz, v = -u0 cos kx sin ky
+    cos kz, w = 0, rho = 1 + 3u0^2/16 (cos 2kx + cos 2ky)(cos 2kz + 2) at cell
+    centres, bit for bit after rounding to float32"""
+    nx, ny, nz = dims
+    u0 = 0.05
+    z, y, x = np.meshgrid(np.arange(nz) + 0.5, np.arange(ny) + 0.5, np.arange(nx) + 0.5,
+                          indexing="ij")
+    X, Y, Z = 2 * np.pi / nx * x, 2 * np.pi / ny * y, 2 * np.pi / nz * z
+    u = u0 * np.sin(X) * np.cos(Y) * np.cos(Z)
+    v = -u0 * np.cos(X) * np.sin(Y) * np.cos(Z)
+    rho = 1.0 + (3.0 * u0 * u0 / 16.0) * (np.cos(2 * X) + np.cos(2 * Y)) * (np.cos(2 * Z) + 2.0)
+    got = wl.taylor_green_3d(nx, ny, nz, u0)
+    for g, r in zip(got, (rho, u, v, np.zeros_like(u))):
+        assert np.array_equal(g, r.reshape(-1).astype(np.float32))
