@@ -132,7 +132,7 @@ def run_method(case, method, steps, warmup, band=0):
     gbps = case.fluid() * b_lup / (per * 1e-3) / 1e9
     steps_per_launch = 2 if pair else 1
     kernel = ("k_pop_pair" if pair else "k_pop_step") if method != 1 else (
-        "k_home_tiled" if variant != 0 else "k_home_step")
+        "k_home_step" if variant == 0 else ("k_home_tiled" if variant <= 3 else "k_home_tiled2"))
     return {"MLUPS": round(case.n * steps / (ms * 1e-3) / 1e6, 1),
             "ms_per_step": round(per, 5), "B_per_LUP": b_lup, "variant": variant,
             "kernel": kernel, "launch_ms": round(per * steps_per_launch, 5),

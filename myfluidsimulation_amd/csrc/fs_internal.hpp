@@ -156,9 +156,11 @@ struct HomeLaunch {
   const uint32_t* nbm;  // bit q: the periodic neighbour c + e_q is solid
   const float* inlet;
 };
-// HOME kernel variants: the plain per-cell gather and three tile shapes
-// (64 x TY columns, ZC-plane z-chunks)
-constexpr int kHomePlain = 0, kHomeT4Z16 = 1, kHomeT8Z16 = 2, kHomeT8Z32 = 3, kHomeVariants = 4;
+// HOME kernel variants: the plain per-cell gather, three tile shapes (64 x TY
+// columns, ZC-plane z-chunks) of the first tiled form and the same three of
+// the second (home_tile2)
+constexpr int kHomePlain = 0, kHomeT4Z16 = 1, kHomeT8Z16 = 2, kHomeT8Z32 = 3, kHome2T4Z16 = 4,
+              kHome2T8Z16 = 5, kHome2T8Z32 = 6, kHomeVariants = 7;
 bool home_variant_ok(const Geo& g, int variant);
 int home_default_variant(const Geo& g);
 // variant < 0: the default for this grid

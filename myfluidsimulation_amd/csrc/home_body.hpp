@@ -307,3 +307,339 @@ __device__ __forceinline__ void home_tile(const Geo& g, const Phys& ph, const Ho
 }  // namespace
 
 }  // namespace fs
+
+namespace fs {
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Tiled HOME step, second form (home_tile2).  Same scheme and the same
+// per-value and per-cell arithmetic as home_tile, organised for fewer vector
+// instructions per lattice update:
+//  * the values that wait for the plane above (a three-plane ring), the
+//    prefetched moments and the flags live in fixed register slots that
+//    rotate once per plane;
+//  * the three directions that stay in the cell's own column (rest, +z, -z)
+//    never go through LDS: 16 LDS planes instead of 19;
+//  * each wave has one halo role, so no wave runs several halo branches:
+//    wave 0 the row below the tile, wave 1 the row above, wave 2 the left
+//    and right columns (one code path for both, the sign of e_x selected per
+//    lane) and the four corners (one direction each);
+//  * a row (one wave: TX = 64) whose 64 output cells are all plain fluid
+//    (no solid neighbour, inlet, outlet) takes a path without the per-lane
+//    flag handling.
+// ---------------------------------------------------------------------------
+// LDS plane of each direction with (e_x, e_y) != (0, 0); -1: kept in registers
+constexpr int kLds16[19] = {-1, 0, 1, 2, 3, -1, -1, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15};
+
+template <int TX, int TY>
+struct TileLds16 {
+  static constexpr int LW = TX + 2, LH = TY + 2, PL = LW * LH;
+  __device__ static __forceinline__ int at(int slot, int row, int col) {
+    return slot * PL + row * LW + col;
+  }
+};
+
+// emit<Q>'s arithmetic for a direction whose x component is sign sx (a
+// lane value): e_x = +1 gives (ux, sxy, sxz) as they are, e_x = -1 their
+// negations, which enter every expression exactly as the compile-time signs
+// do (negation is exact).  Y, Z: the other two components (compile time).
+template <int CY, int CZ>
+__device__ __forceinline__ float emit_sx(const HomeSrc& h, float sux, float ssxy, float ssxz) {
+  constexpr float w = (CY == 0 && CZ == 0) ? 1.0f / 18.0f : 1.0f / 36.0f;
+  constexpr float w45 = 4.5f * w;
+  const float wr = (CY == 0 && CZ == 0) ? h.wr1 : h.wr7;
+  float eu;
+  if constexpr (CY == 0 && CZ == 0)
+    eu = sux;
+  else if constexpr (CZ == 0)
+    eu = sux + sg<CY>(h.uy);
+  else
+    eu = sux + sg<CZ>(h.uz);
+  const float feq = wr * (((1.0f + 3.0f * eu) + (4.5f * eu) * eu) - h.uu15);
+  float contraction = (h.pA[0] + (CY ? h.pA[1] : h.pB[1])) + (CZ ? h.pA[2] : h.pB[2]);
+  if constexpr (CY != 0) contraction += 2.0f * sg<CY>(ssxy);
+  if constexpr (CZ != 0) contraction += 2.0f * sg<CZ>(ssxz);
+  return feq + h.omm * (w45 * contraction);
+}
+
+// the in-plane diagonal (sx, sy, 0) with both signs lane values
+__device__ __forceinline__ float emit_corner(const HomeSrc& h, float sux, float suy, float ssxy) {
+  constexpr float w45 = 4.5f * (1.0f / 36.0f);
+  const float eu = sux + suy;
+  const float feq = h.wr7 * (((1.0f + 3.0f * eu) + (4.5f * eu) * eu) - h.uu15);
+  float contraction = (h.pA[0] + h.pA[1]) + h.pB[2];
+  contraction += 2.0f * ssxy;
+  return feq + h.omm * (w45 * contraction);
+}
+
+template <int TX, int TY, int ZC, int BB>
+__device__ __forceinline__ void home_tile2(const Geo& g, const Phys& ph, const HomeLaunch& a,
+                                           int bx, int by, int bz, float* fs) {
+  static_assert(TX == 64, "one wave per tile row");
+  static_assert(TY >= 3, "three waves carry the halo");
+  using L = TileLds16<TX, TY>;
+  const int tid = threadIdx.x;
+  const int lx = tid % TX, ly = tid / TX;  // ly: the wave
+  const int x0 = bx * TX, y0 = by * TY, z0 = bz * ZC;
+  const int TXe = min(TX, g.nx - x0), TYe = min(TY, g.ny - y0), ZCe = min(ZC, g.nz - z0);
+  const bool active = lx < TXe && ly < TYe;
+  const int x = x0 + lx, y = y0 + ly;
+  const size_t P = g.P;
+  const float omm = 1.0f - ph.omegaL;
+
+  // halo role of this lane: 0 row below, 1 row above, 2 left / right column,
+  // 3 corner, -1 none; (hx, hy) tile-relative
+  int role = -1, hx = 0, hy = 0, hsgn = 1, hsgny = 1;
+  if (ly == 0 && lx < TXe) {
+    role = 0, hx = lx, hy = -1;
+  } else if (ly == 1 && lx < TXe) {
+    role = 1, hx = lx, hy = TYe;
+  } else if (ly == 2) {
+    if (lx < 2 * TYe) {
+      role = 2;
+      const bool left = lx < TYe;
+      hx = left ? -1 : TXe;
+      hy = left ? lx : lx - TYe;
+      hsgn = left ? 1 : -1;  // e_x of the directions that enter the tile
+    } else if (lx < 2 * TYe + 4) {
+      role = 3;
+      const int c = lx - 2 * TYe;  // 0 (-1,-1) 1 (TXe,-1) 2 (-1,TYe) 3 (TXe,TYe)
+      hx = (c & 1) ? TXe : -1;
+      hy = (c & 2) ? TYe : -1;
+      hsgn = (c & 1) ? -1 : 1;
+      hsgny = (c & 2) ? -1 : 1;
+    }
+  }
+  const int gx = (x0 + hx + g.nx) % g.nx, gy = (y0 + hy + g.ny) % g.ny;
+  // corner c's direction (sx, sy, 0) is q = 7 + c; its LDS plane 4 + c
+  const int cornerSlot = 4 + ((hsgn < 0) ? 1 : 0) + ((hsgny < 0) ? 2 : 0);
+  // the left column writes the planes of q = 1, 7, 9, 11, 13 (0, 4, 6, 8, 10),
+  // the right column those of 2, 8, 10, 12, 14 (one further each)
+  const int colSlot0 = hsgn > 0 ? 0 : 1;
+
+  Mom Mm[3], Hm[3];
+  uint8_t Fl[3] = {0, 0, 0};
+  float Ar[3][5], Br[3][9];
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+#pragma unroll
+    for (int k = 0; k < 10; ++k) Mm[s].v[k] = Hm[s].v[k] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) Ar[s][k] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Br[s][k] = 0.0f;
+  }
+
+  auto load_plane = [&](int p, Mom& mc, uint8_t& fc, Mom& mh) {
+    const int pw = (p + g.nz) % g.nz;
+    if (active) {
+      const uint32_t c = (uint32_t(pw) * g.ny + y) * g.nx + x;
+#pragma unroll
+      for (int k = 0; k < 10; ++k) mc.v[k] = a.m[size_t(k) * P + c];
+      fc = a.flags[c];
+    }
+    if (role >= 0) {
+      const uint32_t c = (uint32_t(pw) * g.ny + gy) * g.nx + gx;
+#pragma unroll
+      for (int k = 0; k < 10; ++k) mh.v[k] = a.m[size_t(k) * P + c];
+    }
+  };
+  load_plane(z0 - 1, Mm[0], Fl[0], Hm[0]);
+  const int last = z0 + ZCe;
+
+  auto step = [&](auto phc, int p) {
+    constexpr int PH = decltype(phc)::value;
+    constexpr int NX = (PH + 1) % 3, PV = (PH + 2) % 3;
+    const Mom& cur = Mm[PH];
+    // the output cell of this plane step is (x, y, p-1): its flags, and for
+    // an inlet its velocity, for a near-wall cell its solid-neighbour mask
+    // (loads issued here, so the reconstruction hides their latency)
+    const uint8_t flO = Fl[PV];
+    float uinFin = 0.0f;
+    if (active && (flO & F_INLET)) uinFin = a.inlet[size_t((p - 1 + g.nz) % g.nz) * g.ny + y];
+    uint32_t nbmFin = 0u;
+    if (active && (flO & F_NEAR))
+      nbmFin = a.nbm[(uint32_t((p - 1 + g.nz) % g.nz) * g.ny + y) * g.nx + x];
+    if (p < last) load_plane(p + 1, Mm[NX], Fl[NX], Hm[NX]);
+    float f0 = 0.0f, f5 = 0.0f, f6 = 0.0f;
+    if (active) {
+      const HomeSrc h(cur, omm);
+      for_q<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (kLds16[q] >= 0) fs[L::at(kLds16[q], ly + 1, lx + 1)] = h.template emit<q>();
+      });
+      f0 = h.template emit<0>();
+      f5 = h.template emit<5>();
+      f6 = h.template emit<6>();
+    }
+    if (role >= 0) {
+      const HomeSrc h(Hm[PH], omm);
+      const int row = hy + 1, col = hx + 1;
+      if (role == 0) {  // e_y = +1: q = 3, 7, 8, 15, 17
+        fs[L::at(kLds16[3], row, col)] = h.template emit<3>();
+        fs[L::at(kLds16[7], row, col)] = h.template emit<7>();
+        fs[L::at(kLds16[8], row, col)] = h.template emit<8>();
+        fs[L::at(kLds16[15], row, col)] = h.template emit<15>();
+        fs[L::at(kLds16[17], row, col)] = h.template emit<17>();
+      } else if (role == 1) {  // e_y = -1: q = 4, 9, 10, 16, 18
+        fs[L::at(kLds16[4], row, col)] = h.template emit<4>();
+        fs[L::at(kLds16[9], row, col)] = h.template emit<9>();
+        fs[L::at(kLds16[10], row, col)] = h.template emit<10>();
+        fs[L::at(kLds16[16], row, col)] = h.template emit<16>();
+        fs[L::at(kLds16[18], row, col)] = h.template emit<18>();
+      } else if (role == 2) {  // e_x = hsgn: (s,0,0) (s,1,0) (s,-1,0) (s,0,1) (s,0,-1)
+        const bool pos = hsgn > 0;
+        const float sux = pos ? h.ux : -h.ux, ssxy = pos ? h.sxy : -h.sxy,
+                    ssxz = pos ? h.sxz : -h.sxz;
+        fs[L::at(colSlot0 + 0, row, col)] = emit_sx<0, 0>(h, sux, ssxy, ssxz);
+        fs[L::at(colSlot0 + 4, row, col)] = emit_sx<1, 0>(h, sux, ssxy, ssxz);
+        fs[L::at(colSlot0 + 6, row, col)] = emit_sx<-1, 0>(h, sux, ssxy, ssxz);
+        fs[L::at(colSlot0 + 8, row, col)] = emit_sx<0, 1>(h, sux, ssxy, ssxz);
+        fs[L::at(colSlot0 + 10, row, col)] = emit_sx<0, -1>(h, sux, ssxy, ssxz);
+      } else {  // corner: (sx, sy, 0)
+        const float sux = hsgn > 0 ? h.ux : -h.ux, suy = hsgny > 0 ? h.uy : -h.uy;
+        const float ssxy = (hsgn * hsgny) > 0 ? h.sxy : -h.sxy;
+        fs[L::at(cornerSlot, row, col)] = emit_corner(h, sux, suy, ssxy);
+      }
+    }
+    __syncthreads();
+    // incoming values of this plane: A (e_z = +1) wait for output p+1, B
+    // (e_z = 0) for output p, C (e_z = -1) go to output p-1 now
+    float Cd[5];
+    if (active) {
+      float* A = Ar[PH];
+      float* B = Br[PH];
+      for_q<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        float v;
+        if constexpr (q == 0)
+          v = f0;
+        else if constexpr (q == 5)
+          v = f5;
+        else if constexpr (q == 6)
+          v = f6;
+        else
+          v = fs[L::at(kLds16[q], ly - CY[q] + 1, lx - CX[q] + 1)];
+        if constexpr (CZ[q] > 0) A[kSlot[q]] = v;
+        if constexpr (CZ[q] == 0) B[kSlot[q]] = v;
+        if constexpr (CZ[q] < 0) Cd[kSlot[q]] = v;
+      });
+    }
+    const int o = p - 1;
+    const bool fin = active && o >= z0 && o < last;
+    const float* Ao = Ar[NX];  // from plane o-1 (two steps back)
+    const float* Bo = Br[PV];  // from plane o
+    auto gather = [&](float (&f)[19]) {
+      for_q<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (CZ[q] > 0) f[q] = Ao[kSlot[q]];
+        if constexpr (CZ[q] == 0) f[q] = Bo[kSlot[q]];
+        if constexpr (CZ[q] < 0) f[q] = Cd[kSlot[q]];
+      });
+    };
+    auto moments = [&](const float (&f)[19], float (&out)[10]) {
+      float r = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
+      float Mxx = 0.0f, Mxy = 0.0f, Mxz = 0.0f, Myy = 0.0f, Myz = 0.0f, Mzz = 0.0f;
+      for_q<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int ca = CX[q], cb = CY[q], cc = CZ[q];
+        const float fq = f[q];
+        r += fq;
+        acc1<ca>(jx, fq);
+        acc1<cb>(jy, fq);
+        acc1<cc>(jz, fq);
+        acc1<ca * ca>(Mxx, fq);
+        acc1<ca * cb>(Mxy, fq);
+        acc1<ca * cc>(Mxz, fq);
+        acc1<cb * cb>(Myy, fq);
+        acc1<cb * cc>(Myz, fq);
+        acc1<cc * cc>(Mzz, fq);
+      });
+      home_finish(r, jx, jy, jz, Mxx, Mxy, Mxz, Myy, Myz, Mzz, ph, out);
+    };
+    const int ow = (o + g.nz) % g.nz;
+    const bool plain = fin && (flO & (F_SOLID | F_INLET | F_OUTLET | F_NEAR)) == 0;
+    if (__ballot(plain) == ~0ull) {
+      // the whole row is plain fluid
+      float f[19], out[10];
+      gather(f);
+      moments(f, out);
+      const uint32_t co = (uint32_t(ow) * g.ny + y) * g.nx + x;
+#pragma unroll
+      for (int k = 0; k < 10; ++k) a.mn[size_t(k) * P + co] = out[k];
+    } else {
+      const uint32_t co = (uint32_t(ow) * g.ny + (fin ? y : 0)) * g.nx + (fin ? x : 0);
+      const uint8_t fl = fin ? flO : uint8_t(0);
+      const bool fluid = fin && (fl & (F_SOLID | F_INLET)) == 0;
+      const bool anyNear = __ballot(fluid && (fl & F_NEAR) != 0) != 0;
+      const bool outlet = fin && (fl & F_OUTLET) != 0;
+      const bool anyOutlet = __ballot(outlet) != 0;
+      float out[10];
+#pragma unroll
+      for (int k = 0; k < 10; ++k) out[k] = 0.0f;
+      if (fin) {
+        if (fl & F_SOLID) {
+          out[0] = 1.0f;
+        } else if (fl & F_INLET) {
+          out[0] = 1.0f;
+          out[1] = uinFin;
+        } else {
+          float f[19];
+          gather(f);
+          if (anyNear) {
+            // c - e_q is the neighbour along e_opp(q)
+            const Mom& mPrev = Mm[PV];
+            for_q<1, 19>([&](auto qc) {
+              constexpr int q = decltype(qc)::value;
+              if ((nbmFin >> OPP[q]) & 1u)
+                f[q] = home_rec<BB == 0 ? OPP[q] : OPP_REF[q]>(mPrev, omm);
+            });
+          }
+          moments(f, out);
+        }
+      }
+      // zero-gradient outlet: the new moments of x-1, the previous lane (an
+      // outlet never starts a tile row: the launchers check it)
+      if (anyOutlet) {
+#pragma unroll
+        for (int k = 0; k < 10; ++k) {
+          const float left = __shfl_up(out[k], 1);
+          if (outlet) out[k] = left;
+        }
+      }
+      if (fin) {
+#pragma unroll
+        for (int k = 0; k < 10; ++k) a.mn[size_t(k) * P + co] = out[k];
+      }
+    }
+    __syncthreads();
+  };
+
+  // one loop body on fixed slots (0: this plane, 1: the prefetch, 2: the
+  // plane below); the slots rotate after each plane.  (A body unrolled over
+  // the three slot phases needs no rotation but took 170 VGPRs, three waves
+  // per SIMD, and ran 15% slower.)
+  using I0 = std::integral_constant<int, 0>;
+  for (int p = z0 - 1; p <= last; ++p) {
+    step(I0{}, p);
+    const Mom tm = Mm[0];
+    Mm[0] = Mm[1];
+    Mm[2] = tm;
+    Hm[0] = Hm[1];
+    const uint8_t tf = Fl[0];
+    Fl[0] = Fl[1];
+    Fl[2] = tf;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      Ar[1][k] = Ar[2][k];
+      Ar[2][k] = Ar[0][k];
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Br[2][k] = Br[0][k];
+  }
+}
+
+}  // namespace
+
+}  // namespace fs

@@ -66,6 +66,20 @@ __global__ void __launch_bounds__(TX * TY) k_home_tiled(Geo g, Phys ph, HomeLaun
   home_tile<TX, TY, ZC, BB>(g, ph, a, bx, rest % tilesY, rest / tilesY, fs);
 }
 
+template <int TX, int TY, int ZC, int BB>
+__global__ void __launch_bounds__(TX * TY) k_home_tiled2(Geo g, Phys ph, HomeLaunch a) {
+  __shared__ float fs[16 * TileLds16<TX, TY>::PL];
+  const int tilesX = (g.nx + TX - 1) / TX;
+  // the XCD-contiguous tile order of k_home_tiled
+  uint32_t L0 = blockIdx.x;
+  const uint32_t total = gridDim.x;
+  if ((total & 7u) == 0) L0 = (L0 & 7u) * (total >> 3) + (L0 >> 3);
+  const int tilesY = (g.ny + TY - 1) / TY;
+  const int bx = int(L0 % uint32_t(tilesX));
+  const int rest = int(L0 / uint32_t(tilesX));
+  home_tile2<TX, TY, ZC, BB>(g, ph, a, bx, rest % tilesY, rest / tilesY, fs);
+}
+
 // kern_reset_mom / kern_reinit_from_macro (lbm3d_home.cu:83-137): both buffers
 __global__ void __launch_bounds__(256) k_home_fill(Geo g, float* m, float* mn, const float* rho,
                                                    const float* ux, const float* uy,
@@ -84,15 +98,26 @@ __global__ void __launch_bounds__(256) k_home_fill(Geo g, float* m, float* mn, c
 
 inline dim3 lattice_grid(const Geo& g) { return dim3((g.nxy + 255) / 256, g.nz, 1); }
 
-template <int TY, int ZC>
+template <int TY, int ZC, bool SECOND = false>
 hipError_t launch_tiled(const Geo& g, const Phys& ph, const HomeLaunch& a, hipStream_t st) {
   constexpr int TX = 64;
   const unsigned blocks =
       unsigned(((g.nx + TX - 1) / TX) * ((g.ny + TY - 1) / TY) * ((g.nz + ZC - 1) / ZC));
-  if (ph.bounce == 0)
-    hipLaunchKernelGGL((k_home_tiled<TX, TY, ZC, 0>), dim3(blocks), dim3(TX * TY), 0, st, g, ph, a);
-  else
-    hipLaunchKernelGGL((k_home_tiled<TX, TY, ZC, 1>), dim3(blocks), dim3(TX * TY), 0, st, g, ph, a);
+  if constexpr (SECOND) {
+    if (ph.bounce == 0)
+      hipLaunchKernelGGL((k_home_tiled2<TX, TY, ZC, 0>), dim3(blocks), dim3(TX * TY), 0, st, g,
+                         ph, a);
+    else
+      hipLaunchKernelGGL((k_home_tiled2<TX, TY, ZC, 1>), dim3(blocks), dim3(TX * TY), 0, st, g,
+                         ph, a);
+  } else {
+    if (ph.bounce == 0)
+      hipLaunchKernelGGL((k_home_tiled<TX, TY, ZC, 0>), dim3(blocks), dim3(TX * TY), 0, st, g, ph,
+                         a);
+    else
+      hipLaunchKernelGGL((k_home_tiled<TX, TY, ZC, 1>), dim3(blocks), dim3(TX * TY), 0, st, g, ph,
+                         a);
+  }
   return hipGetLastError();
 }
 
@@ -107,17 +132,20 @@ bool home_variant_ok(const Geo& g, int variant) {
 }
 
 int home_default_variant(const Geo& g) {
-  if (!home_variant_ok(g, kHomeT8Z32)) return kHomePlain;
+  if (!home_variant_ok(g, kHome2T8Z32)) return kHomePlain;
   // a workgroup walks its whole z-chunk: below 512 tiles of 64x4x16 (about
   // 128^3, measured) too few of them fill the chip and the per-cell gather
   // is faster
   const long tiles4 = long((g.nx + 63) / 64) * ((g.ny + 3) / 4) * ((g.nz + 15) / 16);
   if (tiles4 < 512) return kHomePlain;
   // deep chunks cut the re-computed halo planes, but only pay when they
-  // still leave enough workgroups (>= 4 per CU) to fill the chip
+  // still leave enough workgroups (>= 4 per CU) to fill the chip; measured
+  // (second form against the first): 512^3 52.3 vs 51.6 GLUPS, 256^3 46.7 vs
+  // 43.6, 768x256x256 channel 50.9 vs 48.1; 384x128x128 channel (64x4x16)
+  // 38.0 vs 35.0
   const long tiles8 = long((g.nx + 63) / 64) * ((g.ny + 7) / 8);
-  if (g.nz >= 128 && tiles8 * ((g.nz + 31) / 32) >= 1024) return kHomeT8Z32;
-  return kHomeT4Z16;
+  if (g.nz >= 128 && tiles8 * ((g.nz + 31) / 32) >= 1024) return kHome2T8Z32;
+  return kHome2T4Z16;
 }
 
 hipError_t launch_home_step(const Geo& g, const Phys& ph, const HomeLaunch& a, int variant,
@@ -127,6 +155,9 @@ hipError_t launch_home_step(const Geo& g, const Phys& ph, const HomeLaunch& a, i
     case kHomeT4Z16: return launch_tiled<4, 16>(g, ph, a, st);
     case kHomeT8Z16: return launch_tiled<8, 16>(g, ph, a, st);
     case kHomeT8Z32: return launch_tiled<8, 32>(g, ph, a, st);
+    case kHome2T4Z16: return launch_tiled<4, 16, true>(g, ph, a, st);
+    case kHome2T8Z16: return launch_tiled<8, 16, true>(g, ph, a, st);
+    case kHome2T8Z32: return launch_tiled<8, 32, true>(g, ph, a, st);
     default: break;
   }
   if (ph.bounce == 0)
