@@ -9,9 +9,11 @@
 // and small ones share the chip.  The per-tile work is exactly the
 // single-domain kernels' (pop_body.hpp, pair_body.hpp, home_body.hpp), so
 // every domain's fields equal a single-domain run bit for bit.  Population
-// domains advance two steps per launch (k_batch_pair, 32 x 8 tiles: the
-// sweep's edges of 32..128 cells waste fewer lanes than with 64-wide rows),
-// or one (k_batch_pop) for an odd step count.
+// domains advance two steps per launch (k_batch_pair: 64 x 4 tiles where nx
+// is a multiple of 64, else 32 x 8 tiles, which waste fewer lanes than
+// 64-wide rows on the sweep's other edges of 48..112 cells), or one
+// (k_batch_pop) for an odd step count.  HOME domains use the second tiled
+// form (home_tile2) on the same two tile shapes.
 #include "home_body.hpp"
 #include "pair_body.hpp"
 #include "pop_body.hpp"
@@ -21,7 +23,22 @@ namespace fs {
 namespace {
 
 constexpr int kBatchHomeTX = 32, kBatchHomeTY = 8, kBatchHomeZC = 16;
+constexpr int kBatchHomeWideTX = 64, kBatchHomeWideTY = 4;  // nx a multiple of 64
+constexpr int kBatchHomeLds =
+    16 * (TileLds16<kBatchHomeWideTX, kBatchHomeWideTY>::PL >
+                  TileLds16<kBatchHomeTX, kBatchHomeTY>::PL
+              ? TileLds16<kBatchHomeWideTX, kBatchHomeWideTY>::PL
+              : TileLds16<kBatchHomeTX, kBatchHomeTY>::PL);
+static_assert(kBatchHomeWideTX * kBatchHomeWideTY == kBatchHomeTX * kBatchHomeTY, "one block");
+// population tiles: 32 x 8, or 64 x 4 for domains whose nx is a multiple of
+// 64 (full 64-wide rows); both are 256 threads, so one kernel runs either
 constexpr int kBatchPairTX = 32, kBatchPairTY = 8, kBatchPairZC = 16;
+constexpr int kBatchWideTX = 64, kBatchWideTY = 4;
+constexpr int kBatchPairLds =
+    19 * (TileLds<kBatchWideTX, kBatchWideTY>::PL > TileLds<kBatchPairTX, kBatchPairTY>::PL
+              ? TileLds<kBatchWideTX, kBatchWideTY>::PL
+              : TileLds<kBatchPairTX, kBatchPairTY>::PL);
+static_assert(kBatchWideTX * kBatchWideTY == kBatchPairTX * kBatchPairTY, "one block size");
 
 template <bool ODD, bool MACRO>
 __global__ void __launch_bounds__(256) k_batch_pop(const BatchPopDomain* __restrict__ doms,
@@ -53,7 +70,7 @@ template <bool MACRO>
 __global__ void __launch_bounds__(kBatchPairTX * kBatchPairTY)
     k_batch_pair(const BatchPopDomain* __restrict__ doms, const uint16_t* __restrict__ tile_dom,
                  uint32_t ntiles, int cur) {
-  __shared__ float lds[19 * TileLds<kBatchPairTX, kBatchPairTY>::PL];
+  __shared__ float lds[kBatchPairLds];
   // every thread runs the same trip count: pair_tile synchronises the group
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const BatchPopDomain& D = doms[tile_dom[t]];
@@ -71,19 +88,28 @@ __global__ void __launch_bounds__(kBatchPairTX * kBatchPairTY)
     a.uz = D.a.uz;
     a.inlet = D.a.inlet;
     const int bx = rest % D.pairTilesX, by = rest / D.pairTilesX;
-    if (D.hybrid)
-      pair_tile<kBatchPairTX, kBatchPairTY, kBatchPairZC, true, MACRO>(D.g, D.ph, a, bx, by, bz,
-                                                                       lds);
-    else
-      pair_tile<kBatchPairTX, kBatchPairTY, kBatchPairZC, false, MACRO>(D.g, D.ph, a, bx, by, bz,
-                                                                        lds);
+    if (D.pairWide) {
+      if (D.hybrid)
+        pair_tile<kBatchWideTX, kBatchWideTY, kBatchPairZC, true, MACRO>(D.g, D.ph, a, bx, by, bz,
+                                                                         lds);
+      else
+        pair_tile<kBatchWideTX, kBatchWideTY, kBatchPairZC, false, MACRO>(D.g, D.ph, a, bx, by,
+                                                                          bz, lds);
+    } else {
+      if (D.hybrid)
+        pair_tile<kBatchPairTX, kBatchPairTY, kBatchPairZC, true, MACRO>(D.g, D.ph, a, bx, by, bz,
+                                                                         lds);
+      else
+        pair_tile<kBatchPairTX, kBatchPairTY, kBatchPairZC, false, MACRO>(D.g, D.ph, a, bx, by,
+                                                                          bz, lds);
+    }
   }
 }
 
 __global__ void __launch_bounds__(kBatchHomeTX * kBatchHomeTY)
     k_batch_home(const BatchHomeDomain* __restrict__ doms, const uint16_t* __restrict__ tile_dom,
                  uint32_t ntiles, int cur) {
-  __shared__ float fs[19 * TileLds<kBatchHomeTX, kBatchHomeTY>::PL];
+  __shared__ float fs[kBatchHomeLds];
   // every thread runs the same trip count: home_tile synchronises the group
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const BatchHomeDomain& D = doms[tile_dom[t]];
@@ -96,28 +122,44 @@ __global__ void __launch_bounds__(kBatchHomeTX * kBatchHomeTY)
     a.flags = D.flags;
     a.nbm = D.nbm;
     a.inlet = D.inlet;
-    if (D.ph.bounce == 0)
-      home_tile<kBatchHomeTX, kBatchHomeTY, kBatchHomeZC, 0>(D.g, D.ph, a, rest % D.tilesX,
-                                                             rest / D.tilesX, bz, fs);
-    else
-      home_tile<kBatchHomeTX, kBatchHomeTY, kBatchHomeZC, 1>(D.g, D.ph, a, rest % D.tilesX,
-                                                             rest / D.tilesX, bz, fs);
+    const int bx = rest % D.tilesX, by = rest / D.tilesX;
+    if (D.wide) {
+      if (D.ph.bounce == 0)
+        home_tile2<kBatchHomeWideTX, kBatchHomeWideTY, kBatchHomeZC, 0>(D.g, D.ph, a, bx, by, bz,
+                                                                         fs);
+      else
+        home_tile2<kBatchHomeWideTX, kBatchHomeWideTY, kBatchHomeZC, 1>(D.g, D.ph, a, bx, by, bz,
+                                                                         fs);
+    } else {
+      if (D.ph.bounce == 0)
+        home_tile2<kBatchHomeTX, kBatchHomeTY, kBatchHomeZC, 0>(D.g, D.ph, a, bx, by, bz, fs);
+      else
+        home_tile2<kBatchHomeTX, kBatchHomeTY, kBatchHomeZC, 1>(D.g, D.ph, a, bx, by, bz, fs);
+    }
   }
 }
 
 }  // namespace
 
+bool batch_home_wide(const Geo& g) { return g.nx % kBatchHomeWideTX == 0; }
+
 void batch_home_tiling(const Geo& g, int& tilesX, int& tilesY, int& chunks) {
-  tilesX = (g.nx + kBatchHomeTX - 1) / kBatchHomeTX;
-  tilesY = (g.ny + kBatchHomeTY - 1) / kBatchHomeTY;
+  const bool wide = batch_home_wide(g);
+  const int tx = wide ? kBatchHomeWideTX : kBatchHomeTX, ty = wide ? kBatchHomeWideTY : kBatchHomeTY;
+  tilesX = (g.nx + tx - 1) / tx;
+  tilesY = (g.ny + ty - 1) / ty;
   chunks = (g.nz + kBatchHomeZC - 1) / kBatchHomeZC;
 }
 
 int batch_home_tile_width() { return kBatchHomeTX; }
 
+bool batch_pair_wide(const Geo& g) { return g.nx % kBatchWideTX == 0; }
+
 void batch_pair_tiling(const Geo& g, int& tilesX, int& tilesY, int& chunks) {
-  tilesX = (g.nx + kBatchPairTX - 1) / kBatchPairTX;
-  tilesY = (g.ny + kBatchPairTY - 1) / kBatchPairTY;
+  const bool wide = batch_pair_wide(g);
+  const int tx = wide ? kBatchWideTX : kBatchPairTX, ty = wide ? kBatchWideTY : kBatchPairTY;
+  tilesX = (g.nx + tx - 1) / tx;
+  tilesY = (g.ny + ty - 1) / ty;
   chunks = (g.nz + kBatchPairZC - 1) / kBatchPairZC;
 }
 

@@ -301,6 +301,7 @@ int fs_batch_create(const fs_domain_desc* descs, int ndomains, fs_batch** out) {
         h.tile0 = uint32_t(homeTile.size());
         int ty = 0;
         fs::batch_home_tiling(D.g, h.tilesX, ty, h.chunks);
+        h.wide = fs::batch_home_wide(D.g) ? 1 : 0;
         const uint32_t nt = uint32_t(h.tilesX) * uint32_t(ty) * uint32_t(h.chunks);
         D.home = int(b->homeTab.size());
         // work-list entries index the HOME descriptor table, not the domain list
@@ -317,6 +318,7 @@ int fs_batch_create(const fs_domain_desc* descs, int ndomains, fs_batch** out) {
         p.tilesPerPlane = (D.g.nxy + 255) / 256;
         p.pairTile0 = uint32_t(pairTile.size());
         fs::batch_pair_tiling(D.g, p.pairTilesX, p.pairTilesY, p.pairChunks);
+        p.pairWide = fs::batch_pair_wide(D.g) ? 1 : 0;
         D.pop = int(b->popTab.size());
         // work-list entries index the populations descriptor table
         popTile.insert(popTile.end(), size_t(p.tilesPerPlane) * D.g.nz, uint16_t(D.pop));

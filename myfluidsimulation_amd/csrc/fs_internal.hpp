@@ -192,6 +192,7 @@ struct BatchPopDomain {
   uint32_t tilesPerPlane;  // 256-cell tiles per z-plane
   uint32_t pairTile0;      // first two-step work-list tile
   int pairTilesX, pairTilesY, pairChunks;
+  int pairWide;  // two-step tiles 64 x 4 (nx a multiple of 64) instead of 32 x 8
 };
 struct BatchHomeDomain {
   Geo g;
@@ -203,9 +204,14 @@ struct BatchHomeDomain {
   const float* inlet;
   uint32_t tile0;
   int tilesX, chunks;
+  int wide;  // 64 x 4 tiles (nx a multiple of 64) instead of 32 x 8
 };
+bool batch_home_wide(const Geo& g);
 void batch_home_tiling(const Geo& g, int& tilesX, int& tilesY, int& chunks);
 int batch_home_tile_width();
+// two-step tile shape of a population domain: 64 x 4 when nx is a multiple
+// of 64, else 32 x 8 (both 256 threads)
+bool batch_pair_wide(const Geo& g);
 void batch_pair_tiling(const Geo& g, int& tilesX, int& tilesY, int& chunks);
 int batch_pair_tile_width();
 // blocks <= 0: one workgroup per tile; otherwise a persistent grid of that size.

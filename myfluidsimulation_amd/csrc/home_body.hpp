@@ -321,13 +321,13 @@ namespace {
 //    rotate once per plane;
 //  * the three directions that stay in the cell's own column (rest, +z, -z)
 //    never go through LDS: 16 LDS planes instead of 19;
-//  * each wave has one halo role, so no wave runs several halo branches:
-//    wave 0 the row below the tile, wave 1 the row above, wave 2 the left
-//    and right columns (one code path for both, the sign of e_x selected per
-//    lane) and the four corners (one direction each);
-//  * a row (one wave: TX = 64) whose 64 output cells are all plain fluid
-//    (no solid neighbour, inlet, outlet) takes a path without the per-lane
-//    flag handling.
+//  * each tile row has one halo role: row 0 the row below the tile, row 1
+//    the row above, row 2 the left and right columns (one code path for
+//    both, the sign of e_x selected per lane) and the four corners (one
+//    direction each); with 64-wide tiles no wave runs two halo branches;
+//  * a wave (one 64-wide row or two 32-wide ones) whose 64 output cells are
+//    all plain fluid (no solid neighbour, inlet, outlet) takes a path without
+//    the per-lane flag handling.
 // ---------------------------------------------------------------------------
 // LDS plane of each direction with (e_x, e_y) != (0, 0); -1: kept in registers
 constexpr int kLds16[19] = {-1, 0, 1, 2, 3, -1, -1, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15};
@@ -376,11 +376,12 @@ __device__ __forceinline__ float emit_corner(const HomeSrc& h, float sux, float 
 template <int TX, int TY, int ZC, int BB>
 __device__ __forceinline__ void home_tile2(const Geo& g, const Phys& ph, const HomeLaunch& a,
                                            int bx, int by, int bz, float* fs) {
-  static_assert(TX == 64, "one wave per tile row");
-  static_assert(TY >= 3, "three waves carry the halo");
+  // rows 0, 1 and 2 of the tile carry the halo (one or two rows per wave)
+  static_assert(TX == 64 || TX == 32, "a tile row is a wave or half of one");
+  static_assert(TY >= 3 && 2 * TY + 4 <= TX, "the columns and corners fit row 2");
   using L = TileLds16<TX, TY>;
   const int tid = threadIdx.x;
-  const int lx = tid % TX, ly = tid / TX;  // ly: the wave
+  const int lx = tid % TX, ly = tid / TX;
   const int x0 = bx * TX, y0 = by * TY, z0 = bz * ZC;
   const int TXe = min(TX, g.nx - x0), TYe = min(TY, g.ny - y0), ZCe = min(ZC, g.nz - z0);
   const bool active = lx < TXe && ly < TYe;
