@@ -8,8 +8,9 @@ Default workload ("roofline"): the large single-domain roofline runs, FP32.
            icosphere of radius 40 cells written as binary STL and voxelized
            on the GPU) with BGK, HOME and Hybrid(B0) bands 1, 2 and 4;
   extras   the 384x128x128 obstacle channel (radius 20; Hybrid band 2), a
-           shorter run of the 64-domain batched sweep and the geometry path
-           (voxelize / band ms per call) -- skipped with --no-extras;
+           shorter run of the 64-domain batched sweep, the geometry path
+           (voxelize / band ms per call) and the 64^3 Taylor-Green energy
+           history against the analytic decay -- skipped with --no-extras;
   cpu_baseline  the FP32 CPU oracle on the 512^3 vortex (and the 768 channel
            in cpu_baseline_channel768): 5 untimed + 20 timed steps.
 Each method runs W untimed then K timed steps (defaults 50 and 200); one
@@ -326,6 +327,34 @@ def adjoint_bench(method, n, steps, segment=0):
             "grad_finite": bool(np.isfinite(g).all()), "grad_norm": float(np.linalg.norm(g))}
 
 
+def tgv64_energy(steps=1000, every=100):
+    """The 64^3 Taylor-Green case (tau 0.6, u0 0.05) on the GPU: kinetic
+    energy every `every` steps against the linear decay (u0^2/8)
+    exp(-6 nu k^2 t), per method, and a low-amplitude control (u0 0.005)
+    that stays in the linear regime."""
+    from myfluidsimulation_amd import Solver
+    from myfluidsimulation_amd import workloads as wl
+
+    n, tau = 64, 0.6
+    out = {}
+    for u0, tag in ((0.05, ""), (0.005, "_u0_0.005")):
+        fields = wl.taylor_green_3d(n, n, n, u0)
+        for m in (0, 1, 2):
+            with Solver(m, n, n, n, tau) as s:
+                s.reinit_from_macro(*fields)
+                dev = []
+                for k in range(steps // every):
+                    s.step(every)
+                    t = every * (k + 1) - (0 if m == 1 else 1)  # BGK / Hybrid: one-step lag
+                    e = wl.kinetic_energy(*s.macros())
+                    dev.append(round(e / float(wl.tgv_energy_analytic(t, u0, tau, n)) - 1.0, 5))
+            out[NAMES[m] + tag] = dev
+    return {"steps": steps, "every": every,
+            "E_over_analytic_minus_1": out,
+            "note": "the analytic form is the linear decay; at u0 0.05 (Re ~15) nonlinear transfer "
+                    "lowers the energy (FP64 oracle alike), at u0 0.005 it follows to ~1%"}
+
+
 VOXEL_CASES = [  # (label, nx, ny, nz, centre, radius, refine)
     ("channel384", 384, 128, 128, (96, 64, 64), 20.0, 1),
     ("channel384_amr2", 384, 128, 128, (96, 64, 64), 20.0, 2),
@@ -577,6 +606,7 @@ def main():
         line["batched"]["workload"] = b["config"]["workload"]
         line["geometry"] = voxel_bench(3, cpu=not args.no_cpu,
                                        labels=("channel384", "channel768"))["cases"]
+        line["tgv64_energy"] = tgv64_energy()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(case512, warmup_steps=5, timed_steps=20)
         line["cpu_baseline_channel768"] = cpu_baseline(case768, warmup_steps=5, timed_steps=20)

@@ -301,6 +301,12 @@ int fs_band(const uint8_t* host_solid, int nx, int ny, int nz, int d0,
 int fs_band_noalloc(const uint8_t* d_solid, int nx, int ny, int nz, int d0,
                     uint8_t* d_band, void* hip_stream);
 
+/* Extension: the geometry calls (fs_voxelize, fs_band) and the VTK writers
+ * keep their device scratch and pinned host staging between calls (one set
+ * per device, grown to the largest grid seen); this frees them.  A later
+ * call allocates them again.  These calls are serialised per device. */
+int fs_geometry_release(void);
+
 /* ===================== fields and output ================================ */
 
 /* launch_speed (field_ops.cu:25-28) */

@@ -97,6 +97,7 @@ SIGNATURES = {
     "fs_voxelize": (_i, [_p, _i, _p, _p, _i, _i, _i, _f, _f, _f, _f, _i, _f, _p]),
     "fs_band": (_i, [_p, _i, _i, _i, _i, _p]),
     "fs_band_noalloc": (_i, [_p, _i, _i, _i, _i, _p, _p]),
+    "fs_geometry_release": (_i, []),
     "fs_speed": (_i, [_p, _p, _p, _p, _sz]),
     "fs_diff": (_i, [_p, _p, _p, _sz]),
     "fs_shift": (_i, [_p, _f, _p, _sz]),

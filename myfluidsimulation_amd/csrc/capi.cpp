@@ -767,33 +767,43 @@ int fs_voxelize(const float* tri_xyz, int ntri, const float* bbmin3, const float
     std::vector<fs::TriF> tris(static_cast<size_t>(ntri));
     for (int t = 0; t < ntri; ++t)
       for (int j = 0; j < 9; ++j) tris[t].v[j] = sc * tri_xyz[size_t(t) * 9 + j] + trans[j % 3];
-    fs::TriF* d_tris = nullptr;
-    uint8_t* d_fine = nullptr;
-    uint8_t* d_out = nullptr;
-    auto cleanup = [&]() {
-      (void)hipFree(d_tris);
-      (void)hipFree(d_fine);
-      (void)hipFree(d_out);
-    };
+    // device scratch and pinned staging kept between calls; the solid mask
+    // crosses PCIe as packed bits
+    std::unique_lock<std::mutex> lock;
+    fs::GeomScratch* gs = fs::geom_scratch(lock);
+    if (!gs) return fail(FS_ERR_DEVICE, "no current HIP device");
+    const size_t n = size_t(nx) * ny * nz, nw = (n + 31) / 32;
+    void *d_tris = nullptr, *d_fine = nullptr, *d_vis = nullptr, *d_chg = nullptr,
+         *d_words = nullptr, *d_out = nullptr, *h_words = nullptr, *h_chg = nullptr;
+    hipError_t e = gs->device(0, sizeof(fs::TriF) * std::max(1, ntri), &d_tris);
+    if (e == hipSuccess) e = gs->device(1, nf, &d_fine);
+    if (e == hipSuccess) e = gs->device(2, nf, &d_vis);
+    if (e == hipSuccess) e = gs->device(3, sizeof(int), &d_chg);
+    if (e == hipSuccess) e = gs->device(4, sizeof(uint32_t) * nw, &d_words);
+    if (e == hipSuccess && r > 1) e = gs->device(5, n, &d_out);
+    if (e == hipSuccess) e = gs->pinned(0, sizeof(uint32_t) * nw, &h_words);
+    if (e == hipSuccess) e = gs->pinned(1, sizeof(int), &h_chg);
+    if (e != hipSuccess)
+      return fail(FS_ERR_ALLOC, "voxelize scratch allocation failed: %s", hipGetErrorString(e));
     hipStream_t st = nullptr;
-    hipError_t e = hipSuccess;
-    if (ntri > 0) e = hipMalloc((void**)&d_tris, sizeof(fs::TriF) * ntri);
-    if (e == hipSuccess) e = hipMalloc((void**)&d_fine, nf);
-    if (e == hipSuccess && r > 1) e = hipMalloc((void**)&d_out, size_t(nx) * ny * nz);
-    if (e != hipSuccess) {
-      cleanup();
-      return fail(FS_ERR_ALLOC, "device allocation failed: %s", hipGetErrorString(e));
-    }
     if (ntri > 0)
-      e = hipMemcpy(d_tris, tris.data(), sizeof(fs::TriF) * ntri, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = fs::voxelize_device(d_tris, ntri, nx * r, ny * r, nz * r, d_fine, st);
-    if (e == hipSuccess && r > 1)
-      e = fs::downsample_device(d_fine, r, nx, ny, nz, threshold, d_out, st);
+      e = hipMemcpyAsync(d_tris, tris.data(), sizeof(fs::TriF) * ntri, hipMemcpyHostToDevice, st);
+    uint32_t* words = static_cast<uint32_t*>(d_words);
     if (e == hipSuccess)
-      e = hipMemcpy(mask_out, r > 1 ? d_out : d_fine, size_t(nx) * ny * nz,
-                    hipMemcpyDeviceToHost);
-    cleanup();
+      e = fs::voxelize_device(static_cast<fs::TriF*>(d_tris), ntri, nx * r, ny * r, nz * r,
+                              static_cast<uint8_t*>(d_fine), static_cast<uint8_t*>(d_vis),
+                              static_cast<int*>(d_chg), static_cast<int*>(h_chg),
+                              r > 1 ? nullptr : words, st);
+    if (e == hipSuccess && r > 1) {
+      e = fs::downsample_device(static_cast<uint8_t*>(d_fine), r, nx, ny, nz, threshold,
+                                static_cast<uint8_t*>(d_out), st);
+      if (e == hipSuccess) e = fs::pack_bits_device(static_cast<uint8_t*>(d_out), n, words, st);
+    }
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(h_words, words, sizeof(uint32_t) * nw, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return fail(FS_ERR_DEVICE, "voxelize: %s", hipGetErrorString(e));
+    fs::unpack_bits_host(static_cast<const uint32_t*>(h_words), n, mask_out);
     return FS_OK;
   });
 }
@@ -804,21 +814,39 @@ int fs_band(const uint8_t* solid, int nx, int ny, int nz, int d0, uint8_t* band_
   const size_t n = size_t(nx) * ny * nz;
   if (n >= (size_t(1) << 31)) return fail(FS_ERR_INVALID, "grid of %zu cells exceeds 2^31", n);
   if (int rc = fs::runtime_check()) return rc;
-  uint8_t* d_solid = nullptr;
-  uint8_t* d_band = nullptr;
-  uint8_t* d_scr = nullptr;
-  hipError_t e = hipMalloc((void**)&d_solid, n);
-  if (e == hipSuccess) e = hipMalloc((void**)&d_band, n);
-  const int dd = d0 < 0 ? 0 : d0;
-  if (e == hipSuccess) e = hipMalloc((void**)&d_scr, fs::band_scratch_bytes(nx, ny, nz, dd));
-  if (e == hipSuccess) e = hipMemcpy(d_solid, solid, n, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = fs::band_device(d_solid, nx, ny, nz, dd, d_band, d_scr, nullptr);
-  if (e == hipSuccess) e = hipMemcpy(band_out, d_band, n, hipMemcpyDeviceToHost);
-  (void)hipFree(d_solid);
-  (void)hipFree(d_band);
-  (void)hipFree(d_scr);
-  if (e != hipSuccess) return fail(FS_ERR_DEVICE, "band: %s", hipGetErrorString(e));
-  return FS_OK;
+  return guarded([&]() -> int {
+    std::unique_lock<std::mutex> lock;
+    fs::GeomScratch* gs = fs::geom_scratch(lock);
+    if (!gs) return fail(FS_ERR_DEVICE, "no current HIP device");
+    const int dd = d0 < 0 ? 0 : d0;
+    const size_t nw = (n + 31) / 32;
+    void *d_solid = nullptr, *d_band = nullptr, *d_scr = nullptr, *d_words = nullptr,
+         *h_words = nullptr;
+    hipError_t e = gs->device(1, n, &d_solid);
+    if (e == hipSuccess) e = gs->device(7, n, &d_band);
+    if (e == hipSuccess) e = gs->device(6, fs::band_scratch_bytes(nx, ny, nz, dd), &d_scr);
+    if (e == hipSuccess) e = gs->device(4, sizeof(uint32_t) * nw, &d_words);
+    if (e == hipSuccess) e = gs->pinned(0, sizeof(uint32_t) * nw, &h_words);
+    if (e != hipSuccess)
+      return fail(FS_ERR_ALLOC, "band scratch allocation failed: %s", hipGetErrorString(e));
+    uint32_t* hw = static_cast<uint32_t*>(h_words);
+    uint32_t* words = static_cast<uint32_t*>(d_words);
+    hipStream_t st = nullptr;
+    // the solid mask and the band cross PCIe as packed bits
+    fs::pack_bits_host(solid, n, hw);
+    e = hipMemcpyAsync(words, hw, sizeof(uint32_t) * nw, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = fs::unpack_bits_device(words, n, static_cast<uint8_t*>(d_solid), st);
+    if (e == hipSuccess)
+      e = fs::band_device(static_cast<uint8_t*>(d_solid), nx, ny, nz, dd,
+                          static_cast<uint8_t*>(d_band), static_cast<uint8_t*>(d_scr), st);
+    if (e == hipSuccess) e = fs::pack_bits_device(static_cast<uint8_t*>(d_band), n, words, st);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(hw, words, sizeof(uint32_t) * nw, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return fail(FS_ERR_DEVICE, "band: %s", hipGetErrorString(e));
+    fs::unpack_bits_host(hw, n, band_out);
+    return FS_OK;
+  });
 }
 
 // ------------------------------------------------------------------ fields
@@ -869,10 +897,18 @@ int write_scalar(const char* path, const float* d_field, int nx, int ny, int nz,
     return fail(FS_ERR_INVALID, "bad VTK arguments");
   return guarded([&]() -> int {
     const size_t n = size_t(nx) * ny * nz;
-    std::vector<float> h(n);
-    FS_HIP(hipMemcpy(h.data(), d_field, n * sizeof(float), hipMemcpyDeviceToHost));
-    const bool ok = binary ? fs::vtk_scalar_binary(path, h.data(), nx, ny, nz)
-                           : fs::vtk_scalar(path, h.data(), nx, ny, nz);
+    // device -> pinned staging (kept between calls), ordered after earlier
+    // work as the reference's cudaMemcpy (moment_vtk_writer.cu:12)
+    std::unique_lock<std::mutex> lock;
+    fs::GeomScratch* gs = fs::geom_scratch(lock);
+    if (!gs) return fail(FS_ERR_DEVICE, "no current HIP device");
+    void* stage = nullptr;
+    FS_HIP(gs->pinned(2, n * sizeof(float), &stage));
+    float* h = static_cast<float*>(stage);
+    FS_HIP(hipMemcpyAsync(h, d_field, n * sizeof(float), hipMemcpyDeviceToHost, nullptr));
+    FS_HIP(hipStreamSynchronize(nullptr));
+    const bool ok = binary ? fs::vtk_scalar_binary(path, h, nx, ny, nz)
+                           : fs::vtk_scalar(path, h, nx, ny, nz);
     if (!ok) return fail(FS_ERR_IO, "failed to write VTK file %s", path);
     return FS_OK;
   });
@@ -884,11 +920,19 @@ int write_vector(const char* path, const float* a, const float* b, const float* 
     return fail(FS_ERR_INVALID, "bad VTK arguments");
   return guarded([&]() -> int {
     const size_t n = size_t(nx) * ny * nz;
-    std::vector<float> h(3 * n);
-    FS_HIP(hipMemcpy(h.data(), a, n * sizeof(float), hipMemcpyDeviceToHost));
-    FS_HIP(hipMemcpy(h.data() + n, b, n * sizeof(float), hipMemcpyDeviceToHost));
-    FS_HIP(hipMemcpy(h.data() + 2 * n, c, n * sizeof(float), hipMemcpyDeviceToHost));
-    const float *x = h.data(), *y = h.data() + n, *z = h.data() + 2 * n;
+    // the three components -> pinned staging, three asynchronous copies and
+    // one wait (moment_vtk_writer.cu:33-35 copies them one after the other)
+    std::unique_lock<std::mutex> lock;
+    fs::GeomScratch* gs = fs::geom_scratch(lock);
+    if (!gs) return fail(FS_ERR_DEVICE, "no current HIP device");
+    void* stage = nullptr;
+    FS_HIP(gs->pinned(2, 3 * n * sizeof(float), &stage));
+    float* h = static_cast<float*>(stage);
+    FS_HIP(hipMemcpyAsync(h, a, n * sizeof(float), hipMemcpyDeviceToHost, nullptr));
+    FS_HIP(hipMemcpyAsync(h + n, b, n * sizeof(float), hipMemcpyDeviceToHost, nullptr));
+    FS_HIP(hipMemcpyAsync(h + 2 * n, c, n * sizeof(float), hipMemcpyDeviceToHost, nullptr));
+    FS_HIP(hipStreamSynchronize(nullptr));
+    const float *x = h, *y = h + n, *z = h + 2 * n;
     const bool ok = binary ? fs::vtk_vector_binary(path, x, y, z, nx, ny, nz)
                            : fs::vtk_vector(path, x, y, z, nx, ny, nz);
     if (!ok) return fail(FS_ERR_IO, "failed to write VTK file %s", path);

@@ -16,6 +16,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <mutex>
+
 namespace fs {
 
 // D3Q19 velocity set in the reference order (lbm3d_legacy.cu:16-32).
@@ -289,10 +291,39 @@ hipError_t launch_copy_probe(void* dst, const void* src, size_t bytes, hipStream
 struct TriF {
   float v[9];
 };
+// surface + exterior fill into d_out (bytes); vis: n bytes of scratch,
+// changed: one device int, h_changed: one host int (pinned); with d_words
+// non-null the final solid mask is written there as packed bits
+// ((n + 31) / 32 words, bit j of word w = cell 32w + j) and d_out keeps the
+// surface only.  Enqueued on st; it synchronises st once per sweep round.
 hipError_t voxelize_device(const TriF* d_tris, int ntri, int nx, int ny, int nz, uint8_t* d_out,
+                           uint8_t* vis, int* changed, int* h_changed, uint32_t* d_words,
                            hipStream_t st);
+// bytes (nonzero = 1) <-> packed bits, as above
+hipError_t pack_bits_device(const uint8_t* d_in, size_t n, uint32_t* d_words, hipStream_t st);
+hipError_t unpack_bits_device(const uint32_t* d_words, size_t n, uint8_t* d_out, hipStream_t st);
 hipError_t downsample_device(const uint8_t* d_fine, int refine, int nx, int ny, int nz,
                              float threshold, uint8_t* d_out, hipStream_t st);
+// Device scratch and pinned host staging of the geometry calls (stage.cpp),
+// one set per device, grown on demand and kept between calls; a call holds
+// the mutex (geom_scratch's lock) while it uses them.
+struct GeomScratch {
+  static constexpr int kSlots = 8;
+  std::mutex mu;
+  void* dev[kSlots] = {};
+  size_t devCap[kSlots] = {};
+  void* host[kSlots] = {};
+  size_t hostCap[kSlots] = {};
+  hipError_t device(int slot, size_t bytes, void** out);
+  hipError_t pinned(int slot, size_t bytes, void** out);
+  void release();
+};
+// the current device's set, locked; nullptr if the device is unknown
+GeomScratch* geom_scratch(std::unique_lock<std::mutex>& lock);
+// host bit (un)packing of masks, bit j of word w = cell 32w + j (threads)
+void unpack_bits_host(const uint32_t* words, size_t n, uint8_t* out);
+void pack_bits_host(const uint8_t* in, size_t n, uint32_t* words);
+
 // d_scratch: band_scratch_bytes(nx, ny, nz, d0) bytes (one byte per cell
 // while at most 254 distance rings are needed, an int per cell beyond)
 size_t band_scratch_bytes(int nx, int ny, int nz, int d0);

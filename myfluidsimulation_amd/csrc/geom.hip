@@ -105,18 +105,39 @@ __global__ void __launch_bounds__(256) k_surface(const TriF* __restrict__ tris, 
   }
 }
 
-// exterior seeds: non-surface cells on the six faces (cells are fewer than
-// 2^31, so the index arithmetic is 32-bit)
+// exterior seeds: non-surface cells on the six faces.  Sixteen cells per
+// thread, loaded and stored as one 16-byte vector (the buffers are
+// 256-byte aligned; the last thread of an odd-sized grid takes its bytes one
+// by one).  Cells are fewer than 2^31, so the index arithmetic is 32-bit.
 __global__ void __launch_bounds__(256) k_seed(const uint8_t* __restrict__ surf, int nx, int ny,
                                               int nz, uint8_t* __restrict__ vis) {
-  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
   const uint32_t n = uint32_t(nx) * uint32_t(ny) * uint32_t(nz);
-  if (i >= n) return;
-  const uint32_t row = i / uint32_t(nx);
-  const int x = int(i - row * uint32_t(nx));
-  const int y = int(row % uint32_t(ny)), z = int(row / uint32_t(ny));
-  const bool face = x == 0 || y == 0 || z == 0 || x == nx - 1 || y == ny - 1 || z == nz - 1;
-  vis[i] = (face && !surf[i]) ? 1 : 0;
+  const uint32_t i0 = (blockIdx.x * 256u + threadIdx.x) * 16u;
+  if (i0 >= n) return;
+  uint32_t row = i0 / uint32_t(nx);
+  int x = int(i0 - row * uint32_t(nx));
+  int y = int(row % uint32_t(ny)), z = int(row / uint32_t(ny));
+  auto seed = [&](uint8_t sv) {
+    const bool face = x == 0 || y == 0 || z == 0 || x == nx - 1 || y == ny - 1 || z == nz - 1;
+    const uint8_t v = (face && !sv) ? 1 : 0;
+    if (++x == nx) {
+      x = 0;
+      if (++y == ny) y = 0, ++z;
+    }
+    return v;
+  };
+  if (i0 + 16u <= n) {
+    union {
+      uint4 v;
+      uint8_t b[16];
+    } in, out;
+    in.v = *reinterpret_cast<const uint4*>(surf + i0);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) out.b[j] = seed(in.b[j]);
+    *reinterpret_cast<uint4*>(vis + i0) = out.v;
+  } else {
+    for (uint32_t i = i0; i < n; ++i) vis[i] = seed(surf[i]);
+  }
 }
 
 // Sets the round's "something changed" flag.  In a round that fills the
@@ -296,11 +317,92 @@ __global__ void __launch_bounds__(256) k_sweep_yz(const uint8_t* __restrict__ su
   if (any) mark_changed(changed);
 }
 
+// solid = surface or not exterior, sixteen cells per thread
 __global__ void __launch_bounds__(256) k_finalize(uint8_t* __restrict__ surf_io,
                                                   const uint8_t* __restrict__ vis, size_t n) {
-  const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
-  if (i >= n) return;
-  surf_io[i] = (surf_io[i] || !vis[i]) ? 1 : 0;
+  const size_t i0 = (size_t(blockIdx.x) * 256 + threadIdx.x) * 16;
+  if (i0 >= n) return;
+  if (i0 + 16 <= n) {
+    union {
+      uint4 v;
+      uint8_t b[16];
+    } s4, v4;
+    s4.v = *reinterpret_cast<const uint4*>(surf_io + i0);
+    v4.v = *reinterpret_cast<const uint4*>(vis + i0);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) s4.b[j] = (s4.b[j] || !v4.b[j]) ? 1 : 0;
+    *reinterpret_cast<uint4*>(surf_io + i0) = s4.v;
+  } else {
+    for (size_t i = i0; i < n; ++i) surf_io[i] = (surf_io[i] || !vis[i]) ? 1 : 0;
+  }
+}
+
+// the same, written as packed bits (bit j of word w: cell 32w + j; bits past
+// n are 0): one word per thread
+__global__ void __launch_bounds__(256) k_finalize_pack(const uint8_t* __restrict__ surf,
+                                                       const uint8_t* __restrict__ vis, size_t n,
+                                                       uint32_t* __restrict__ words) {
+  const size_t w = size_t(blockIdx.x) * 256 + threadIdx.x;
+  const size_t i0 = w * 32;
+  if (i0 >= n) return;
+  uint32_t bits = 0;
+  if (i0 + 32 <= n) {
+    union {
+      uint4 v[2];
+      uint8_t b[32];
+    } s4, v4;
+    s4.v[0] = *reinterpret_cast<const uint4*>(surf + i0);
+    s4.v[1] = *reinterpret_cast<const uint4*>(surf + i0 + 16);
+    v4.v[0] = *reinterpret_cast<const uint4*>(vis + i0);
+    v4.v[1] = *reinterpret_cast<const uint4*>(vis + i0 + 16);
+#pragma unroll
+    for (int j = 0; j < 32; ++j) bits |= uint32_t(s4.b[j] || !v4.b[j]) << j;
+  } else {
+    for (size_t i = i0; i < n; ++i) bits |= uint32_t(surf[i] || !vis[i]) << (i - i0);
+  }
+  words[w] = bits;
+}
+
+// bytes (nonzero = 1) -> packed bits, and back
+__global__ void __launch_bounds__(256) k_pack_bits(const uint8_t* __restrict__ in, size_t n,
+                                                   uint32_t* __restrict__ words) {
+  const size_t w = size_t(blockIdx.x) * 256 + threadIdx.x;
+  const size_t i0 = w * 32;
+  if (i0 >= n) return;
+  uint32_t bits = 0;
+  if (i0 + 32 <= n) {
+    union {
+      uint4 v[2];
+      uint8_t b[32];
+    } b4;
+    b4.v[0] = *reinterpret_cast<const uint4*>(in + i0);
+    b4.v[1] = *reinterpret_cast<const uint4*>(in + i0 + 16);
+#pragma unroll
+    for (int j = 0; j < 32; ++j) bits |= uint32_t(b4.b[j] != 0) << j;
+  } else {
+    for (size_t i = i0; i < n; ++i) bits |= uint32_t(in[i] != 0) << (i - i0);
+  }
+  words[w] = bits;
+}
+
+__global__ void __launch_bounds__(256) k_unpack_bits(const uint32_t* __restrict__ words, size_t n,
+                                                     uint8_t* __restrict__ out) {
+  const size_t w = size_t(blockIdx.x) * 256 + threadIdx.x;
+  const size_t i0 = w * 32;
+  if (i0 >= n) return;
+  const uint32_t bits = words[w];
+  if (i0 + 32 <= n) {
+    union {
+      uint4 v[2];
+      uint8_t b[32];
+    } b4;
+#pragma unroll
+    for (int j = 0; j < 32; ++j) b4.b[j] = uint8_t((bits >> j) & 1u);
+    *reinterpret_cast<uint4*>(out + i0) = b4.v[0];
+    *reinterpret_cast<uint4*>(out + i0 + 16) = b4.v[1];
+  } else {
+    for (size_t i = i0; i < n; ++i) out[i] = uint8_t((bits >> (i - i0)) & 1u);
+  }
 }
 
 // downsample_amr (voxelize.cpp:187-226)
@@ -376,6 +478,7 @@ inline dim3 grid1d(size_t n) { return dim3(unsigned((n + 255) / 256)); }
 }  // namespace
 
 hipError_t voxelize_device(const TriF* d_tris, int ntri, int nx, int ny, int nz, uint8_t* d_out,
+                           uint8_t* vis, int* changed, int* h_changed, uint32_t* d_words,
                            hipStream_t st) {
   const size_t n = size_t(nx) * ny * nz;
   hipError_t e = hipMemsetAsync(d_out, 0, n, st);
@@ -384,14 +487,7 @@ hipError_t voxelize_device(const TriF* d_tris, int ntri, int nx, int ny, int nz,
     hipLaunchKernelGGL(k_surface, dim3(ntri), dim3(256), 0, st, d_tris, nx, ny, nz, d_out);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
-  uint8_t* vis = nullptr;
-  int* changed = nullptr;
-  if ((e = hipMalloc((void**)&vis, n)) != hipSuccess) return e;
-  if ((e = hipMalloc((void**)&changed, sizeof(int))) != hipSuccess) {
-    (void)hipFree(vis);
-    return e;
-  }
-  hipLaunchKernelGGL(k_seed, grid1d(n), dim3(256), 0, st, d_out, nx, ny, nz, vis);
+  hipLaunchKernelGGL(k_seed, grid1d((n + 15) / 16), dim3(256), 0, st, d_out, nx, ny, nz, vis);
   const size_t lines[3] = {size_t(ny) * nz, size_t(nx) * nz, size_t(nx) * ny};
   // every round that changes something grows the set; the set has at most n
   // cells, so the loop ends
@@ -402,21 +498,33 @@ hipError_t voxelize_device(const TriF* d_tris, int ntri, int nx, int ny, int nz,
     for (int ax = 1; ax < 3; ++ax)
       hipLaunchKernelGGL(k_sweep_yz, grid1d(lines[ax]), dim3(256), 0, st, d_out, vis, nx, ny, nz,
                          ax, changed);
-    int h = 0;
-    if ((e = hipMemcpyAsync(&h, changed, sizeof(int), hipMemcpyDeviceToHost, st)) != hipSuccess)
+    if ((e = hipMemcpyAsync(h_changed, changed, sizeof(int), hipMemcpyDeviceToHost, st)) !=
+        hipSuccess)
       break;
     if ((e = hipStreamSynchronize(st)) != hipSuccess) break;
-    if (!h) break;
+    if (!*h_changed) break;
   }
   if (e == hipSuccess) {
-    hipLaunchKernelGGL(k_finalize, grid1d(n), dim3(256), 0, st, d_out, vis, n);
+    if (d_words)
+      hipLaunchKernelGGL(k_finalize_pack, grid1d((n + 31) / 32), dim3(256), 0, st, d_out, vis, n,
+                         d_words);
+    else
+      hipLaunchKernelGGL(k_finalize, grid1d((n + 15) / 16), dim3(256), 0, st, d_out, vis, n);
     e = hipGetLastError();
   }
-  hipError_t es = hipStreamSynchronize(st);
-  if (e == hipSuccess) e = es;
-  (void)hipFree(vis);
-  (void)hipFree(changed);
   return e;
+}
+
+hipError_t pack_bits_device(const uint8_t* d_in, size_t n, uint32_t* d_words, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pack_bits, grid1d((n + 31) / 32), dim3(256), 0, st, d_in, n, d_words);
+  return hipGetLastError();
+}
+
+hipError_t unpack_bits_device(const uint32_t* d_words, size_t n, uint8_t* d_out, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_unpack_bits, grid1d((n + 31) / 32), dim3(256), 0, st, d_words, n, d_out);
+  return hipGetLastError();
 }
 
 hipError_t downsample_device(const uint8_t* d_fine, int refine, int nx, int ny, int nz,

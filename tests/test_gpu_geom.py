@@ -273,3 +273,25 @@ def test_binary_vtk_writer(fs, oracle, tmp_path):
     lines, vals = read_binary_vtk(pv, 3)
     np.testing.assert_array_equal(vals.reshape(n, 3), np.stack([u, v, w], axis=1))
     assert lines[2] == "BINARY" and lines[7] == f"POINT_DATA {n}"
+
+
+def test_geometry_scratch_reuse_and_release(fs, oracle, tmp_path):
+    """fs_voxelize / fs_band keep their device scratch and pinned staging
+    between calls: a large call, a smaller one (odd cell counts, so packed
+    words end mid-word), fs_geometry_release, and the small call again all
+    equal the oracle; masks whose bytes are not 0/1 count as nonzero."""
+    from myfluidsimulation_amd import capi
+
+    path = str(tmp_path / "ico.stl")
+    for nx, ny, nz in ((96, 64, 48), (37, 23, 19)):
+        scale, trans = wl.obstacle_stl(path, nx, ny, nz, (nx / 3, ny / 2, nz / 2), min(ny, nz) / 4,
+                                       seed=nx)
+        tris, bmin, bmax = fs.load_stl(path)
+        g = fs.voxelize(tris, bmin, bmax, nx, ny, nz, scale, trans)
+        assert np.array_equal(g, oracle.voxelize(tris, bmin, bmax, nx, ny, nz, scale, trans))
+        solid = (g * 7).astype(np.uint8)  # nonzero, not 1
+        assert np.array_equal(fs.band(solid, nx, ny, nz, 2), oracle.band(g, nx, ny, nz, 2))
+    capi.call("fs_geometry_release")
+    g2 = fs.voxelize(tris, bmin, bmax, nx, ny, nz, scale, trans)
+    assert np.array_equal(g2, g)
+    assert np.array_equal(fs.band(g2, nx, ny, nz, 3), oracle.band(g2, nx, ny, nz, 3))
