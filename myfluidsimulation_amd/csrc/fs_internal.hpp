@@ -162,7 +162,8 @@ struct HomeLaunch {
 // columns, ZC-plane z-chunks) of the first tiled form and the same three of
 // the second (home_tile2)
 constexpr int kHomePlain = 0, kHomeT4Z16 = 1, kHomeT8Z16 = 2, kHomeT8Z32 = 3, kHome2T4Z16 = 4,
-              kHome2T8Z16 = 5, kHome2T8Z32 = 6, kHomeVariants = 7;
+              kHome2T8Z16 = 5, kHome2T8Z32 = 6, kHome2T16Z32 = 7, kHome2T8Z64 = 8,
+              kHomeVariants = 9;
 bool home_variant_ok(const Geo& g, int variant);
 int home_default_variant(const Geo& g);
 // variant < 0: the default for this grid

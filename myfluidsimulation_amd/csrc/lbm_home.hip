@@ -144,6 +144,10 @@ int home_default_variant(const Geo& g) {
   // 43.6, 768x256x256 channel 50.9 vs 48.1; 384x128x128 channel (64x4x16)
   // 38.0 vs 35.0
   const long tiles8 = long((g.nx + 63) / 64) * ((g.ny + 7) / 8);
+  // 64-plane chunks halve the re-read planes again where the grid is deep
+  // and still gives two workgroups per CU: 512^3 53.6 against 53.1 GLUPS
+  // (32-plane chunks), 256^3 48.1 against 46.7; the 768x256x256 channel ties
+  if (g.nz >= 256 && tiles8 * ((g.nz + 63) / 64) >= 512) return kHome2T8Z64;
   if (g.nz >= 128 && tiles8 * ((g.nz + 31) / 32) >= 1024) return kHome2T8Z32;
   return kHome2T4Z16;
 }
@@ -158,6 +162,8 @@ hipError_t launch_home_step(const Geo& g, const Phys& ph, const HomeLaunch& a, i
     case kHome2T4Z16: return launch_tiled<4, 16, true>(g, ph, a, st);
     case kHome2T8Z16: return launch_tiled<8, 16, true>(g, ph, a, st);
     case kHome2T8Z32: return launch_tiled<8, 32, true>(g, ph, a, st);
+    case kHome2T16Z32: return launch_tiled<16, 32, true>(g, ph, a, st);
+    case kHome2T8Z64: return launch_tiled<8, 64, true>(g, ph, a, st);
     default: break;
   }
   if (ph.bounce == 0)
