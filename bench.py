@@ -9,8 +9,10 @@ Default workload ("roofline"): the large single-domain roofline runs, FP32.
            on the GPU) with BGK, HOME and Hybrid(B0) bands 1, 2 and 4;
   extras   the 384x128x128 obstacle channel (radius 20; Hybrid band 2), a
            shorter run of the 64-domain batched sweep, the geometry path
-           (voxelize / band ms per call) and the 64^3 Taylor-Green energy
-           history against the analytic decay -- skipped with --no-extras;
+           (voxelize / band ms per call), the 64^3 Taylor-Green energy
+           history against the analytic decay and the 128^3 differentiable
+           run (adjoint128 below, all three methods) -- skipped with
+           --no-extras;
   cpu_baseline  the FP32 CPU oracle on the 512^3 vortex (and the 768 channel
            in cpu_baseline_channel768): 5 untimed + 20 timed steps.
 Each method runs W untimed then K timed steps (defaults 50 and 200); one
@@ -607,6 +609,13 @@ def main():
         line["geometry"] = voxel_bench(3, cpu=not args.no_cpu,
                                        labels=("channel384", "channel768"))["cases"]
         line["tgv64_energy"] = tgv64_energy()
+        keep = ("steps", "segment", "checkpoints", "device_GB", "plain_MLUPS", "forward_s",
+                "backward_s", "adjoint_MLUPS", "adjoint_over_plain", "grad_finite")
+        line["adjoint128"] = {
+            "workload": "Taylor-Green 128^3 inverse problem, dJ/du0, T = 2000, sqrt(T) checkpoints",
+            **{name: {k: r[k] for k in keep}
+               for name, r in zip(("bgk", "home", "hybrid"),
+                                  (adjoint_bench(m, 128, 2000) for m in (0, 1, 2)))}}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(case512, warmup_steps=5, timed_steps=20)
         line["cpu_baseline_channel768"] = cpu_baseline(case768, warmup_steps=5, timed_steps=20)
