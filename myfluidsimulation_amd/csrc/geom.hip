@@ -473,6 +473,86 @@ __global__ void __launch_bounds__(256) k_band_mask(const uint8_t* __restrict__ s
   band[i] = (!solid[i] && d != DistT<D>::kUnreached && int(d) <= d0) ? 1 : 0;
 }
 
+// One-byte distances on grids whose rows are whole 16-byte blocks (nx a
+// multiple of 16): sixteen cells per thread, loaded and stored as 16-byte
+// vectors; a block never straddles a row, so its y/z neighbours are the
+// aligned blocks at +-nx and +-nx*ny and its x neighbours the bytes just
+// before and after it.
+union Cells16 {
+  uint4 v;
+  uint8_t b[16];
+};
+
+__global__ void __launch_bounds__(256) k_dist_init16(const uint8_t* __restrict__ solid,
+                                                     uint32_t nblocks, uint8_t* __restrict__ dist) {
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  if (t >= nblocks) return;
+  Cells16 s, d;
+  s.v = reinterpret_cast<const uint4*>(solid)[t];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) d.b[j] = s.b[j] ? uint8_t(0) : DistT<uint8_t>::kUnreached;
+  reinterpret_cast<uint4*>(dist)[t] = d.v;
+}
+
+// ring k over 16-cell blocks: every value read is either unchanged by this
+// ring or goes from "unreached" to k, neither of which equals k-1, so reading
+// the neighbours' blocks while other threads update theirs gives the same
+// ring as the one-cell kernel
+__global__ void __launch_bounds__(256) k_dist_ring16(uint8_t* __restrict__ dist, int nx, int ny,
+                                                     int nz, int k) {
+  const uint32_t nb = uint32_t(nx) / 16u * uint32_t(ny) * uint32_t(nz);
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  if (t >= nb) return;
+  constexpr uint8_t U = DistT<uint8_t>::kUnreached;
+  Cells16 c;
+  c.v = reinterpret_cast<const uint4*>(dist)[t];
+  bool any = false;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) any = any || c.b[j] == U;
+  if (!any) return;
+  const uint32_t i0 = t * 16u;
+  const uint32_t row = i0 / uint32_t(nx);
+  const int x0 = int(i0 - row * uint32_t(nx));
+  const int y = int(row % uint32_t(ny)), z = int(row / uint32_t(ny));
+  const uint32_t by = uint32_t(nx) / 16u, bz = by * uint32_t(ny);
+  const uint8_t p = uint8_t(k - 1);
+  // a neighbour outside the grid (no periodic wrap) never matches
+  Cells16 ym, yp, zm, zp;
+  ym.v = yp.v = zm.v = zp.v = make_uint4(0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu);
+  const uint4* d4 = reinterpret_cast<const uint4*>(dist);
+  if (y > 0) ym.v = d4[t - by];
+  if (y < ny - 1) yp.v = d4[t + by];
+  if (z > 0) zm.v = d4[t - bz];
+  if (z < nz - 1) zp.v = d4[t + bz];
+  const uint8_t left = x0 > 0 ? dist[i0 - 1] : U;
+  const uint8_t right = x0 + 16 < nx ? dist[i0 + 16] : U;
+  Cells16 o = c;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const uint8_t xm = j > 0 ? c.b[j - 1] : left;
+    const uint8_t xp = j < 15 ? c.b[j + 1] : right;
+    const bool hit = xm == p || xp == p || ym.b[j] == p || yp.b[j] == p || zm.b[j] == p ||
+                     zp.b[j] == p;
+    if (c.b[j] == U && hit) o.b[j] = uint8_t(k);
+  }
+  reinterpret_cast<uint4*>(dist)[t] = o.v;
+}
+
+__global__ void __launch_bounds__(256) k_band_mask16(const uint8_t* __restrict__ solid,
+                                                     const uint8_t* __restrict__ dist,
+                                                     uint32_t nblocks, int d0,
+                                                     uint8_t* __restrict__ band) {
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  if (t >= nblocks) return;
+  Cells16 s, d, b;
+  s.v = reinterpret_cast<const uint4*>(solid)[t];
+  d.v = reinterpret_cast<const uint4*>(dist)[t];
+#pragma unroll
+  for (int j = 0; j < 16; ++j)
+    b.b[j] = (!s.b[j] && d.b[j] != DistT<uint8_t>::kUnreached && int(d.b[j]) <= d0) ? 1 : 0;
+  reinterpret_cast<uint4*>(band)[t] = b.v;
+}
+
 inline dim3 grid1d(size_t n) { return dim3(unsigned((n + 255) / 256)); }
 
 }  // namespace
@@ -546,8 +626,23 @@ template <class D>
 hipError_t band_run(const uint8_t* d_solid, int nx, int ny, int nz, int d0, uint8_t* d_band,
                     D* dist, hipStream_t st) {
   const size_t n = size_t(nx) * ny * nz;
-  hipLaunchKernelGGL(k_dist_init<D>, grid1d(n), dim3(256), 0, st, d_solid, n, dist);
   const int rings = band_rings(nx, ny, nz, d0);
+  if constexpr (sizeof(D) == 1) {
+    // 16-byte blocks need 16-byte aligned buffers (device allocations are;
+    // a caller's buffers of fs_band_noalloc may not be)
+    const uintptr_t al = reinterpret_cast<uintptr_t>(d_solid) | reinterpret_cast<uintptr_t>(d_band) |
+                         reinterpret_cast<uintptr_t>(dist);
+    if (nx % 16 == 0 && al % 16 == 0) {
+      const uint32_t nb = uint32_t(n / 16);
+      hipLaunchKernelGGL(k_dist_init16, grid1d(nb), dim3(256), 0, st, d_solid, nb, dist);
+      for (int k = 1; k <= rings; ++k)
+        hipLaunchKernelGGL(k_dist_ring16, grid1d(nb), dim3(256), 0, st, dist, nx, ny, nz, k);
+      hipLaunchKernelGGL(k_band_mask16, grid1d(nb), dim3(256), 0, st, d_solid, dist, nb, d0,
+                         d_band);
+      return hipGetLastError();
+    }
+  }
+  hipLaunchKernelGGL(k_dist_init<D>, grid1d(n), dim3(256), 0, st, d_solid, n, dist);
   for (int k = 1; k <= rings; ++k)
     hipLaunchKernelGGL(k_dist_ring<D>, grid1d(n), dim3(256), 0, st, dist, nx, ny, nz, k);
   hipLaunchKernelGGL(k_band_mask<D>, grid1d(n), dim3(256), 0, st, d_solid, dist, n, d0, d_band);

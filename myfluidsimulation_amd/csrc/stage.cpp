@@ -8,6 +8,7 @@
 // calls' device scratch are kept and grown across calls (one set per
 // device, guarded by a mutex that each geometry call holds throughout), so a
 // repeated call allocates nothing; fs_geometry_release frees them.
+#include <emmintrin.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -134,11 +135,15 @@ void unpack_bits_host(const uint32_t* words, size_t n, uint8_t* out) {
 void pack_bits_host(const uint8_t* in, size_t n, uint32_t* words) {
   const size_t full = n / 32;
   parallel_words(full, [&](size_t b, size_t e) {
+    // 16 bytes at a time: the byte-equal-zero mask, one bit per byte in order
+    const __m128i zero = _mm_setzero_si128();
     for (size_t w = b; w < e; ++w) {
       const uint8_t* p = in + w * 32;
-      uint32_t x = 0;
-      for (int j = 0; j < 32; ++j) x |= uint32_t(p[j] != 0) << j;
-      words[w] = x;
+      const __m128i lo = _mm_loadu_si128(reinterpret_cast<const __m128i*>(p));
+      const __m128i hi = _mm_loadu_si128(reinterpret_cast<const __m128i*>(p + 16));
+      const uint32_t z = uint32_t(_mm_movemask_epi8(_mm_cmpeq_epi8(lo, zero))) |
+                         (uint32_t(_mm_movemask_epi8(_mm_cmpeq_epi8(hi, zero))) << 16);
+      words[w] = ~z;
     }
   });
   if (full * 32 < n) {

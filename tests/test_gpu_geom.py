@@ -134,9 +134,12 @@ def test_voxelize_channel_config(fs, oracle, tmp_path):
     assert 0.7 * vol < g.sum() < 1.4 * vol
 
 
+@pytest.mark.parametrize("nx", [40, 48], ids=["cellwise", "16-cell blocks"])
 @pytest.mark.parametrize("d0", [0, 1, 2, 3, 5])
-def test_band_exact(fs, oracle, d0):
-    nx, ny, nz = 40, 24, 20
+def test_band_exact(fs, oracle, d0, nx):
+    """Rows of 40 cells take the one-cell kernels, rows of 48 (a multiple of
+    16) the 16-cell block kernels."""
+    ny, nz = 24, 20
     rng = np.random.default_rng(d0)
     solid = (rng.random(nx * ny * nz) < 0.02).astype(np.uint8)
     solid.reshape(nz, ny, nx)[:, 0, :] = 1
@@ -145,11 +148,14 @@ def test_band_exact(fs, oracle, d0):
     assert np.array_equal(g, o)
 
 
+@pytest.mark.parametrize("nx", [280, 288])
 @pytest.mark.parametrize("d0", [200, 254, 255, 300])
-def test_band_deep_exact(fs, oracle, d0):
-    """Bands of hundreds of rings on a long thin grid (Manhattan diameter 299):
-    distances are kept in one byte up to 254 rings and in an int beyond."""
-    nx, ny, nz = 280, 12, 10
+def test_band_deep_exact(fs, oracle, d0, nx):
+    """Bands of hundreds of rings on a long thin grid (Manhattan diameter
+    299-307): distances are kept in one byte up to 254 rings (one-cell
+    kernels for rows of 280 cells, 16-cell blocks for 288) and in an int
+    beyond."""
+    ny, nz = 12, 10
     solid = np.zeros(nx * ny * nz, np.uint8)
     solid.reshape(nz, ny, nx)[nz // 2, ny // 2, 3] = 1
     g = fs.band(solid, nx, ny, nz, d0)
