@@ -107,12 +107,9 @@ int build_flags_and_links(fs_adjoint* a, const uint8_t* body) {
   return dev_alloc(a, &a->partials, sizeof(float) * kMaxDragBlocks);
 }
 
+// drag of one step over the body links (there is at least one)
 int drag_step(fs_adjoint* a, const float* state, float* out) {
   fs_solver* s = a->s;
-  if (a->nlinks == 0) {
-    FS_HIP(fs::launch_fill(out, 0.0f, 1, s->stream));
-    return FS_OK;
-  }
   if (a->home)
     FS_HIP(fs::launch_drag_home(s->g, s->ph, state, a->links, a->nlinks, a->partials, a->nblocks,
                                 out, s->stream));
@@ -221,6 +218,9 @@ int fs_adjoint_forward(fs_adjoint* a, const float* rho0, const float* ux0, const
     if (int rc = solver_reinit_async(s, a->init, a->init + P, a->init + 2 * P, a->init + 3 * P))
       return rc;
   }
+  // without body links every step's drag is zero: one fill for all T steps
+  if (drag && a->nlinks == 0) FS_HIP(fs::launch_fill(drag, 0.0f, size_t(a->T), st));
+  float* const dragSteps = a->nlinks ? drag : nullptr;
   for (int n = 0; n < a->T; ++n) {
     if (n % a->S == 0) {
       const int k = n / a->S;
@@ -234,16 +234,16 @@ int fs_adjoint_forward(fs_adjoint* a, const float* rho0, const float* ux0, const
       }
     }
     if (a->home) {
-      if (drag)
-        if (int rc = drag_step(a, s->mom_cur(), drag + n)) return rc;
+      if (dragSteps)
+        if (int rc = drag_step(a, s->mom_cur(), dragSteps + n)) return rc;
       const fs::HomeLaunch hl{s->mbuf[s->cur], s->mbuf[1 - s->cur], s->flags, s->nbm, s->inlet};
       FS_HIP(fs::launch_home_step(g, s->ph, hl, solver_home_variant(s), st));
       s->cur = 1 - s->cur;
     } else {
       FS_HIP(fs::launch_pop_step(g, s->ph, pop_args(s), s->parity != 0, a->hyb, n == a->T - 1, st));
       s->parity ^= 1;
-      if (drag)
-        if (int rc = drag_step(a, s->F, drag + n)) return rc;
+      if (dragSteps)
+        if (int rc = drag_step(a, s->F, dragSteps + n)) return rc;
     }
   }
   float* outs[4] = {out_rho, out_ux, out_uy, out_uz};
