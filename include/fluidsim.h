@@ -150,10 +150,12 @@ int fs_solver_timed_steps(fs_solver* s, int nsteps, float* elapsed_ms);
 /* Extension: kernel variant selection.  HOME has a plain per-cell gather
  * (variant 0) and tiled kernels (1: 64x4 columns x 16 planes, 2: 64x8 x 16,
  * 3: 64x8 x 32; 4, 5, 6: the same shapes in a second form with fewer
- * instructions per update; 7: 64x16 x 32 and 8: 64x8 x 64 in that form).  BGK / Hybrid have a one-step in-place kernel (variant 0)
- * and kernels that advance two steps per launch through LDS (1: 64x4 x 16,
- * 2: 64x8 x 16, 3: 64x8 x 32, 4: 64x4 x 32, 5: 32x8 x 16; a step(n) call
- * with n odd runs one one-step launch).  Every variant gives the same bits.
+ * instructions per update; 7: 64x16 x 32 and 8: 64x8 x 64 in that form).
+ * BGK / Hybrid have a one-step in-place kernel (variant 0) and kernels that
+ * advance two steps per launch through LDS (1: 64x4 x 16, 2: 64x8 x 16,
+ * 3: 64x8 x 32, 4: 64x4 x 32, 5: 32x8 x 16, 6: 64x12 x 16, 7: 64x12 x 32,
+ * 8: 64x12 x 64; a step(n) call with n odd runs one one-step launch).
+ * Every variant gives the same bits.
  * -1 restores the per-grid default (the one-step / plain kernels below about
  * 128^3 cells).  While an outlet cell sits at an x that is a multiple of the
  * variant's tile width (64, or 32 for variant 5) the solver runs variant 0

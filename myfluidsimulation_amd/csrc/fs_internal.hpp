@@ -143,7 +143,7 @@ struct PairLaunch {
 // launch on TX x TY column tiles of ZC planes: 64 x (4,16), (8,16), (8,32),
 // (4,32), and 32 x (8,16)
 constexpr int kPopAA = 0, kPairT4Z16 = 1, kPairT8Z16 = 2, kPairT8Z32 = 3, kPairT4Z32 = 4,
-              kPairX32T8Z16 = 5, kPopVariants = 6;
+              kPairX32T8Z16 = 5, kPairT12Z16 = 6, kPairT12Z32 = 7, kPairT12Z64 = 8, kPopVariants = 9;
 int pop_default_variant(const Geo& g);
 // tile row width of a two-step variant: an outlet may not sit at a multiple
 int pop_variant_tile_width(int variant);

@@ -76,18 +76,23 @@ int pop_default_variant(const Geo& g) {
   // A workgroup walks its whole z-chunk, so a small grid gives too few of
   // them to fill 256 CUs: below 512 tiles of 64x4x16 (measured crossover,
   // about 128^3) the one-step kernel, with one workgroup per 256 cells, is
-  // faster.  Otherwise 64x4x16 tiles (three workgroups per CU), except where
-  // 64x8 tiles with 32-plane chunks (one per CU, the fewest halo rows and
-  // planes re-read) give 1024 to 2047 workgroups: at 256^3 (1024 of them)
-  // 64x4x16 leaves a third-full last round and 64x8x32 is ahead (46.0 against
-  // 42.9 GLUPS), while at 3072 (768x256x256 channel: 38.6 against 40.7) and
-  // 8192 (512^3: 42.5 against 43.7-44.0) 64x4x16 is; the 384x128x128
-  // channel gives only 384 of the former.
+  // faster.
   const long cols = (g.nx + 63) / 64;
   const long tiles4 = cols * ((g.ny + 3) / 4) * ((g.nz + 15) / 16);
   if (tiles4 < 512) return kPopAA;
-  const long tiles8z32 = cols * ((g.ny + 7) / 8) * ((g.nz + 31) / 32);
-  return (tiles8z32 >= 1024 && tiles8z32 < 2048) ? kPairT8Z32 : kPairT4Z16;
+  // 64x12 tiles (768 threads, one workgroup per CU at three waves per SIMD)
+  // re-collide a ring of 156 cells for 768 (64x4: 140 for 256) and re-read
+  // the fewest halo rows, but with one workgroup per CU the last round of
+  // workgroups must not be nearly empty: 64-plane chunks where they give at
+  // least 2048 workgroups (512^3: 2752, 53.3-53.9 GLUPS against 43.3-43.6
+  // for 64x4x16), 32-plane chunks from 512 (768x256x256 channel, 2112: 45.5-45.9
+  // against 43.5-44.5; 256^3, 704: 42.7-43.1 against 41.0 for 64x8x32)
+  const long rows12 = (g.ny + 11) / 12;
+  if (cols * rows12 * ((g.nz + 63) / 64) >= 2048) return kPairT12Z64;
+  if (cols * rows12 * ((g.nz + 31) / 32) >= 512) return kPairT12Z32;
+  // Otherwise 64x4x16 tiles (three workgroups per CU): the 384x128x128
+  // channel (264 workgroups of 64x12x32: 28 GLUPS) takes them (39.0).
+  return kPairT4Z16;
 }
 
 hipError_t launch_pop_pair(const Geo& g, const Phys& ph, const PairLaunch& a, bool hybrid,
@@ -98,6 +103,9 @@ hipError_t launch_pop_pair(const Geo& g, const Phys& ph, const PairLaunch& a, bo
     case kPairT8Z32: return launch_pair<64, 8, 32>(g, ph, a, hybrid, write_macro, st);
     case kPairT4Z32: return launch_pair<64, 4, 32>(g, ph, a, hybrid, write_macro, st);
     case kPairX32T8Z16: return launch_pair<32, 8, 16>(g, ph, a, hybrid, write_macro, st);
+    case kPairT12Z16: return launch_pair<64, 12, 16>(g, ph, a, hybrid, write_macro, st);
+    case kPairT12Z32: return launch_pair<64, 12, 32>(g, ph, a, hybrid, write_macro, st);
+    case kPairT12Z64: return launch_pair<64, 12, 64>(g, ph, a, hybrid, write_macro, st);
     default: return launch_pair<64, 4, 16>(g, ph, a, hybrid, write_macro, st);
   }
 }

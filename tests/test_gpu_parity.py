@@ -290,7 +290,7 @@ def test_pop_variants_and_autotune(fs, oracle, method):
     for k in chunks:
         o.step(k)
         refs.append((o.state(), o.macros()))
-    for variant in (0, 1, 2, 3, 4, 5, -1):
+    for variant in (0, 1, 2, 3, 4, 5, 6, 7, 8, -1):
         with fs.Solver(method, nx, ny, nz, 0.58, (2e-6, 0.0, 0.0)) as s:
             s.set_flags(flags)
             s.set_inlet_profile(inlet)
@@ -301,7 +301,7 @@ def test_pop_variants_and_autotune(fs, oracle, method):
             if variant == -1:
                 before, mac_before = s.state(), s.macros()
                 chosen = s.autotune(3)
-                assert chosen in (0, 1, 2, 3, 4, 5) and s.variant == chosen
+                assert chosen in range(9) and s.variant == chosen
                 np.testing.assert_array_equal(s.state(), before)
                 for g, r in zip(s.macros(), mac_before):  # the macro planes too
                     np.testing.assert_array_equal(g, r)
@@ -331,7 +331,7 @@ def test_pop_pair_tile_shapes(fs, oracle, dims):
         f[:, 0, :] = 1
         f[:, ny - 1, nx // 2:nx // 2 + 3] = 1
         flags = f.reshape(-1)
-    for variant in (1, 2, 3, 4, 5):
+    for variant in (1, 2, 3, 4, 5, 6, 7, 8):
         nxyz = (nx, ny, nz)
         o = oracle.Oracle(0, *nxyz, 0.6, (1e-5, 0.0, 0.0))
         s = fs.Solver(0, *nxyz, 0.6, (1e-5, 0.0, 0.0))
