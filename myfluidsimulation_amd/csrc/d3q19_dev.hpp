@@ -161,6 +161,29 @@ struct HomeSrc {
     if constexpr (b != 0 && c != 0) contraction += 2.0f * sg<b * c>(syz);
     return feq + omm * (w45 * contraction);
   }
+
+  // emit<Q>() and emit<OPP[Q]>() in one go, with the same bits as the two
+  // calls: e_opp = -e_q, so the opposite direction's e.u is -eu, its 3*eu is
+  // -(3*eu) and its (4.5*eu)*eu is the same product (negation is exact and
+  // round-to-nearest is symmetric), 1 + (-t) is 1 - t, and the contraction
+  // (even in e) is the same value.
+  template <int Q>
+  __device__ __forceinline__ void emit_pair(float& fq, float& fo) const {
+    static_assert(Q != 0, "the rest direction has no partner");
+    constexpr int a = Dir<Q>::cx, b = Dir<Q>::cy, c = Dir<Q>::cz;
+    constexpr float w45 = 4.5f * Dir<Q>::w;
+    const float wr = Q <= 6 ? wr1 : wr7;
+    const float eu = edot<Q>(ux, uy, uz);
+    const float t1 = 3.0f * eu;
+    const float t2 = (4.5f * eu) * eu;
+    float contraction = ((a ? pA[0] : pB[0]) + (b ? pA[1] : pB[1])) + (c ? pA[2] : pB[2]);
+    if constexpr (a != 0 && b != 0) contraction += 2.0f * sg<a * b>(sxy);
+    if constexpr (a != 0 && c != 0) contraction += 2.0f * sg<a * c>(sxz);
+    if constexpr (b != 0 && c != 0) contraction += 2.0f * sg<b * c>(syz);
+    const float fneq = omm * (w45 * contraction);
+    fq = wr * (((1.0f + t1) + t2) - uu15) + fneq;
+    fo = wr * (((1.0f - t1) + t2) - uu15) + fneq;
+  }
 };
 
 // Compile-time loop over the 19 directions.

@@ -644,3 +644,338 @@ __device__ __forceinline__ void home_tile2(const Geo& g, const Phys& ph, const H
 }  // namespace
 
 }  // namespace fs
+
+namespace fs {
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Tiled HOME step, third form (home_tile3).  The scheme, LDS layout, halo
+// roles and per-value arithmetic of home_tile2 (so the same bits), with fewer
+// vector instructions per lattice update:
+//  * a tile cell emits each direction together with its opposite
+//    (HomeSrc::emit_pair): the two share e.u, 3*e.u, (4.5*e.u)*e.u and the
+//    contraction;
+//  * the plane walk is unrolled six planes deep, so every register ring
+//    slot (moments and flags: two planes, e_z = 0 values: two, e_z = +1
+//    values: three) is fixed at compile time and no slot is copied;
+//  * a near-wall cell reads its own moments back (from the input buffer, the
+//    plane the tile read two planes earlier) instead of keeping a third
+//    plane of moments in registers;
+//  * the flags of a plane are read with that plane's moments one plane
+//    before the plane's cells are finished.
+// ---------------------------------------------------------------------------
+// Plane k of a moment buffer read or written at a 32-bit byte offset: the
+// plane's base is the same for every lane (scalar registers) and a lane's
+// offset is one 32-bit register, so no lane builds 64-bit addresses.  Every
+// plane must then be under 4 GiB (grids of fewer than 2^30 cells; the
+// launchers check it).
+// The empty asm pins the (uniform) base in scalar registers and keeps the
+// compiler from folding the lane offset into it first, which would give every
+// plane its own 64-bit vector address.
+// (The accesses name the global address space: through the opaque base the
+// compiler could no longer infer it and would emit flat accesses.)
+using gbyte = __attribute__((address_space(1))) char;
+__device__ __forceinline__ float ld_b(const float* base, uint32_t boff) {
+  asm volatile("" : "+s"(base));
+  const gbyte* b = (const gbyte*)(base);
+  return *(const __attribute__((address_space(1))) float*)(b + boff);
+}
+__device__ __forceinline__ void st_b(float* base, uint32_t boff, float v) {
+  asm volatile("" : "+s"(base));
+  gbyte* b = (gbyte*)(base);
+  *(__attribute__((address_space(1))) float*)(b + boff) = v;
+}
+
+template <int TX, int TY, int ZC, int BB>
+__device__ __forceinline__ void home_tile3(const Geo& g, const Phys& ph, const HomeLaunch& a,
+                                           int bx, int by, int bz, float* fs) {
+  static_assert(TX == 64 || TX == 32, "a tile row is a wave or half of one");
+  static_assert(TY >= 3 && 2 * TY + 4 <= TX, "the columns and corners fit row 2");
+  using L = TileLds16<TX, TY>;
+  const int tid = threadIdx.x;
+  const int lx = tid % TX, ly = tid / TX;
+  const int x0 = bx * TX, y0 = by * TY, z0 = bz * ZC;
+  const int TXe = min(TX, g.nx - x0), TYe = min(TY, g.ny - y0), ZCe = min(ZC, g.nz - z0);
+  const bool active = lx < TXe && ly < TYe;
+  const int x = x0 + lx, y = y0 + ly;
+  const size_t P = g.P;
+  const float omm = 1.0f - ph.omegaL;
+
+  // halo role of this lane (as home_tile2): 0 row below, 1 row above, 2 left
+  // / right column, 3 corner, -1 none; (hx, hy) tile-relative
+  int role = -1, hx = 0, hy = 0, hsgn = 1, hsgny = 1;
+  if (ly == 0 && lx < TXe) {
+    role = 0, hx = lx, hy = -1;
+  } else if (ly == 1 && lx < TXe) {
+    role = 1, hx = lx, hy = TYe;
+  } else if (ly == 2) {
+    if (lx < 2 * TYe) {
+      role = 2;
+      const bool left = lx < TYe;
+      hx = left ? -1 : TXe;
+      hy = left ? lx : lx - TYe;
+      hsgn = left ? 1 : -1;
+    } else if (lx < 2 * TYe + 4) {
+      role = 3;
+      const int c = lx - 2 * TYe;
+      hx = (c & 1) ? TXe : -1;
+      hy = (c & 2) ? TYe : -1;
+      hsgn = (c & 1) ? -1 : 1;
+      hsgny = (c & 2) ? -1 : 1;
+    }
+  }
+  const int gx = (x0 + hx + g.nx) % g.nx, gy = (y0 + hy + g.ny) % g.ny;
+  const int cornerSlot = 4 + ((hsgn < 0) ? 1 : 0) + ((hsgny < 0) ? 2 : 0);
+  const int colSlot0 = hsgn > 0 ? 0 : 1;
+  // plane-independent parts of this lane's cell and halo-cell indices
+  const uint32_t cxy = uint32_t(y) * g.nx + x, hxy = uint32_t(gy) * g.nx + gx;
+  const uint32_t cxy4 = cxy * 4u, hxy4 = hxy * 4u, plane4 = g.nxy * 4u;
+
+  Mom Mm[2], Hm[2];
+  uint8_t Fl[2] = {0, 0};
+  float Ar[3][5], Br[2][9];
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+#pragma unroll
+    for (int k = 0; k < 10; ++k) Mm[s].v[k] = Hm[s].v[k] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Br[s][k] = 0.0f;
+  }
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+#pragma unroll
+    for (int k = 0; k < 5; ++k) Ar[s][k] = 0.0f;
+  }
+
+  auto load_plane = [&](int p, Mom& mc, Mom& mh) {
+    const uint32_t pw4 = uint32_t((p + g.nz) % g.nz) * plane4;
+    if (active) {
+#pragma unroll
+      for (int k = 0; k < 10; ++k) mc.v[k] = ld_b(a.m + size_t(k) * P, pw4 + cxy4);
+    }
+    if (role >= 0) {
+#pragma unroll
+      for (int k = 0; k < 10; ++k) mh.v[k] = ld_b(a.m + size_t(k) * P, pw4 + hxy4);
+    }
+  };
+  load_plane(z0 - 1, Mm[0], Hm[0]);
+  const int last = z0 + ZCe;
+
+  // one plane step; K = position of plane p in the six-plane unroll
+  auto step = [&](auto kc, int p) {
+    constexpr int K = decltype(kc)::value;
+    constexpr int S2 = K % 2, N2 = (K + 1) % 2;  // this plane's / the other slot of two
+    constexpr int S3 = K % 3, O3 = (K + 1) % 3;  // this plane's / two planes back, of three
+    const Mom& cur = Mm[S2];
+    const uint32_t pOut = uint32_t((p - 1 + g.nz) % g.nz);
+    // the output cell of this plane step is (x, y, p-1); its flags were read
+    // one plane ago; an inlet's velocity and a near-wall cell's solid
+    // neighbours are read here, ahead of the reconstruction
+    const uint8_t flO = Fl[N2];
+    float uinFin = 0.0f;
+    if (active && (flO & F_INLET)) uinFin = a.inlet[size_t(pOut) * g.ny + y];
+    uint32_t nbmFin = 0u;
+    if (active && (flO & F_NEAR)) nbmFin = a.nbm[pOut * g.nxy + cxy];
+    if (p < last) load_plane(p + 1, Mm[N2], Hm[N2]);
+    if (active) Fl[S2] = a.flags[uint32_t((p + g.nz) % g.nz) * g.nxy + cxy];
+    float f0 = 0.0f, f5 = 0.0f, f6 = 0.0f;
+    if (active) {
+      const HomeSrc h(cur, omm);
+      auto put = [&](auto qc, float v) {
+        constexpr int q = decltype(qc)::value;
+        fs[L::at(kLds16[q], ly + 1, lx + 1)] = v;
+      };
+      using std::integral_constant;
+      float u, v;
+      h.template emit_pair<1>(u, v);
+      put(integral_constant<int, 1>{}, u);
+      put(integral_constant<int, 2>{}, v);
+      h.template emit_pair<3>(u, v);
+      put(integral_constant<int, 3>{}, u);
+      put(integral_constant<int, 4>{}, v);
+      h.template emit_pair<7>(u, v);
+      put(integral_constant<int, 7>{}, u);
+      put(integral_constant<int, 10>{}, v);
+      h.template emit_pair<8>(u, v);
+      put(integral_constant<int, 8>{}, u);
+      put(integral_constant<int, 9>{}, v);
+      h.template emit_pair<11>(u, v);
+      put(integral_constant<int, 11>{}, u);
+      put(integral_constant<int, 14>{}, v);
+      h.template emit_pair<12>(u, v);
+      put(integral_constant<int, 12>{}, u);
+      put(integral_constant<int, 13>{}, v);
+      h.template emit_pair<15>(u, v);
+      put(integral_constant<int, 15>{}, u);
+      put(integral_constant<int, 18>{}, v);
+      h.template emit_pair<16>(u, v);
+      put(integral_constant<int, 16>{}, u);
+      put(integral_constant<int, 17>{}, v);
+      f0 = h.template emit<0>();
+      h.template emit_pair<5>(f5, f6);
+    }
+    if (role >= 0) {
+      const HomeSrc h(Hm[S2], omm);
+      const int row = hy + 1, col = hx + 1;
+      if (role == 0) {  // e_y = +1: q = 3, 7, 8, 15, 17
+        fs[L::at(kLds16[3], row, col)] = h.template emit<3>();
+        fs[L::at(kLds16[7], row, col)] = h.template emit<7>();
+        fs[L::at(kLds16[8], row, col)] = h.template emit<8>();
+        fs[L::at(kLds16[15], row, col)] = h.template emit<15>();
+        fs[L::at(kLds16[17], row, col)] = h.template emit<17>();
+      } else if (role == 1) {  // e_y = -1: q = 4, 9, 10, 16, 18
+        fs[L::at(kLds16[4], row, col)] = h.template emit<4>();
+        fs[L::at(kLds16[9], row, col)] = h.template emit<9>();
+        fs[L::at(kLds16[10], row, col)] = h.template emit<10>();
+        fs[L::at(kLds16[16], row, col)] = h.template emit<16>();
+        fs[L::at(kLds16[18], row, col)] = h.template emit<18>();
+      } else if (role == 2) {  // e_x = hsgn: (s,0,0) (s,1,0) (s,-1,0) (s,0,1) (s,0,-1)
+        const bool pos = hsgn > 0;
+        const float sux = pos ? h.ux : -h.ux, ssxy = pos ? h.sxy : -h.sxy,
+                    ssxz = pos ? h.sxz : -h.sxz;
+        fs[L::at(colSlot0 + 0, row, col)] = emit_sx<0, 0>(h, sux, ssxy, ssxz);
+        fs[L::at(colSlot0 + 4, row, col)] = emit_sx<1, 0>(h, sux, ssxy, ssxz);
+        fs[L::at(colSlot0 + 6, row, col)] = emit_sx<-1, 0>(h, sux, ssxy, ssxz);
+        fs[L::at(colSlot0 + 8, row, col)] = emit_sx<0, 1>(h, sux, ssxy, ssxz);
+        fs[L::at(colSlot0 + 10, row, col)] = emit_sx<0, -1>(h, sux, ssxy, ssxz);
+      } else {  // corner: (sx, sy, 0)
+        const float sux = hsgn > 0 ? h.ux : -h.ux, suy = hsgny > 0 ? h.uy : -h.uy;
+        const float ssxy = (hsgn * hsgny) > 0 ? h.sxy : -h.sxy;
+        fs[L::at(cornerSlot, row, col)] = emit_corner(h, sux, suy, ssxy);
+      }
+    }
+    __syncthreads();
+    // incoming values of this plane: A (e_z = +1) wait for output p+1, B
+    // (e_z = 0) for output p, C (e_z = -1) go to output p-1 now
+    float Cd[5];
+    if (active) {
+      float* A = Ar[S3];
+      float* B = Br[S2];
+      for_q<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        float v;
+        if constexpr (q == 0)
+          v = f0;
+        else if constexpr (q == 5)
+          v = f5;
+        else if constexpr (q == 6)
+          v = f6;
+        else
+          v = fs[L::at(kLds16[q], ly - CY[q] + 1, lx - CX[q] + 1)];
+        if constexpr (CZ[q] > 0) A[kSlot[q]] = v;
+        if constexpr (CZ[q] == 0) B[kSlot[q]] = v;
+        if constexpr (CZ[q] < 0) Cd[kSlot[q]] = v;
+      });
+    }
+    const int o = p - 1;
+    const bool fin = active && o >= z0 && o < last;
+    const float* Ao = Ar[O3];  // from plane o-1 (two steps back)
+    const float* Bo = Br[N2];  // from plane o
+    auto gather = [&](float (&f)[19]) {
+      for_q<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (CZ[q] > 0) f[q] = Ao[kSlot[q]];
+        if constexpr (CZ[q] == 0) f[q] = Bo[kSlot[q]];
+        if constexpr (CZ[q] < 0) f[q] = Cd[kSlot[q]];
+      });
+    };
+    auto moments = [&](const float (&f)[19], float (&out)[10]) {
+      float r = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
+      float Mxx = 0.0f, Mxy = 0.0f, Mxz = 0.0f, Myy = 0.0f, Myz = 0.0f, Mzz = 0.0f;
+      for_q<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int ca = CX[q], cb = CY[q], cc = CZ[q];
+        const float fq = f[q];
+        r += fq;
+        acc1<ca>(jx, fq);
+        acc1<cb>(jy, fq);
+        acc1<cc>(jz, fq);
+        acc1<ca * ca>(Mxx, fq);
+        acc1<ca * cb>(Mxy, fq);
+        acc1<ca * cc>(Mxz, fq);
+        acc1<cb * cb>(Myy, fq);
+        acc1<cb * cc>(Myz, fq);
+        acc1<cc * cc>(Mzz, fq);
+      });
+      home_finish(r, jx, jy, jz, Mxx, Mxy, Mxz, Myy, Myz, Mzz, ph, out);
+    };
+    const bool plain = fin && (flO & (F_SOLID | F_INLET | F_OUTLET | F_NEAR)) == 0;
+    if (__ballot(plain) == ~0ull) {
+      // the whole row is plain fluid
+      float f[19], out[10];
+      gather(f);
+      moments(f, out);
+      const uint32_t co4 = pOut * plane4 + cxy4;
+#pragma unroll
+      for (int k = 0; k < 10; ++k) st_b(a.mn + size_t(k) * P, co4, out[k]);
+    } else {
+      const uint32_t co4 = pOut * plane4 + (fin ? cxy4 : 0u);
+      const uint8_t fl = fin ? flO : uint8_t(0);
+      const bool fluid = fin && (fl & (F_SOLID | F_INLET)) == 0;
+      const bool anyNear = __ballot(fluid && (fl & F_NEAR) != 0) != 0;
+      const bool outlet = fin && (fl & F_OUTLET) != 0;
+      const bool anyOutlet = __ballot(outlet) != 0;
+      float out[10];
+#pragma unroll
+      for (int k = 0; k < 10; ++k) out[k] = 0.0f;
+      if (fin) {
+        if (fl & F_SOLID) {
+          out[0] = 1.0f;
+        } else if (fl & F_INLET) {
+          out[0] = 1.0f;
+          out[1] = uinFin;
+        } else {
+          float f[19];
+          gather(f);
+          if (anyNear && nbmFin != 0u) {
+            // the cell's own moments (plane o of the input buffer); c - e_q
+            // is the neighbour along e_opp(q)
+            Mom mOwn;
+#pragma unroll
+            for (int k = 0; k < 10; ++k) mOwn.v[k] = ld_b(a.m + size_t(k) * P, co4);
+            for_q<1, 19>([&](auto qc) {
+              constexpr int q = decltype(qc)::value;
+              if ((nbmFin >> OPP[q]) & 1u)
+                f[q] = home_rec<BB == 0 ? OPP[q] : OPP_REF[q]>(mOwn, omm);
+            });
+          }
+          moments(f, out);
+        }
+      }
+      // zero-gradient outlet: the new moments of x-1, the previous lane (an
+      // outlet never starts a tile row: the launchers check it)
+      if (anyOutlet) {
+#pragma unroll
+        for (int k = 0; k < 10; ++k) {
+          const float left = __shfl_up(out[k], 1);
+          if (outlet) out[k] = left;
+        }
+      }
+      if (fin) {
+#pragma unroll
+        for (int k = 0; k < 10; ++k) st_b(a.mn + size_t(k) * P, co4, out[k]);
+      }
+    }
+    __syncthreads();
+  };
+
+  using std::integral_constant;
+  for (int p = z0 - 1; p <= last; p += 6) {
+    step(integral_constant<int, 0>{}, p);
+    if (p + 1 > last) break;
+    step(integral_constant<int, 1>{}, p + 1);
+    if (p + 2 > last) break;
+    step(integral_constant<int, 2>{}, p + 2);
+    if (p + 3 > last) break;
+    step(integral_constant<int, 3>{}, p + 3);
+    if (p + 4 > last) break;
+    step(integral_constant<int, 4>{}, p + 4);
+    if (p + 5 > last) break;
+    step(integral_constant<int, 5>{}, p + 5);
+  }
+}
+
+}  // namespace
+
+}  // namespace fs

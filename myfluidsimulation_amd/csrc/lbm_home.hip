@@ -80,6 +80,20 @@ __global__ void __launch_bounds__(TX * TY) k_home_tiled2(Geo g, Phys ph, HomeLau
   home_tile2<TX, TY, ZC, BB>(g, ph, a, bx, rest % tilesY, rest / tilesY, fs);
 }
 
+template <int TX, int TY, int ZC, int BB>
+__global__ void __launch_bounds__(TX * TY) k_home_tiled3(Geo g, Phys ph, HomeLaunch a) {
+  __shared__ float fs[16 * TileLds16<TX, TY>::PL];
+  const int tilesX = (g.nx + TX - 1) / TX;
+  // the XCD-contiguous tile order of k_home_tiled
+  uint32_t L0 = blockIdx.x;
+  const uint32_t total = gridDim.x;
+  if ((total & 7u) == 0) L0 = (L0 & 7u) * (total >> 3) + (L0 >> 3);
+  const int tilesY = (g.ny + TY - 1) / TY;
+  const int bx = int(L0 % uint32_t(tilesX));
+  const int rest = int(L0 / uint32_t(tilesX));
+  home_tile3<TX, TY, ZC, BB>(g, ph, a, bx, rest % tilesY, rest / tilesY, fs);
+}
+
 // kern_reset_mom / kern_reinit_from_macro (lbm3d_home.cu:83-137): both buffers
 __global__ void __launch_bounds__(256) k_home_fill(Geo g, float* m, float* mn, const float* rho,
                                                    const float* ux, const float* uy,
@@ -98,12 +112,19 @@ __global__ void __launch_bounds__(256) k_home_fill(Geo g, float* m, float* mn, c
 
 inline dim3 lattice_grid(const Geo& g) { return dim3((g.nxy + 255) / 256, g.nz, 1); }
 
-template <int TY, int ZC, bool SECOND = false>
+template <int TY, int ZC, int FORM = 1>
 hipError_t launch_tiled(const Geo& g, const Phys& ph, const HomeLaunch& a, hipStream_t st) {
   constexpr int TX = 64;
   const unsigned blocks =
       unsigned(((g.nx + TX - 1) / TX) * ((g.ny + TY - 1) / TY) * ((g.nz + ZC - 1) / ZC));
-  if constexpr (SECOND) {
+  if constexpr (FORM == 3) {
+    if (ph.bounce == 0)
+      hipLaunchKernelGGL((k_home_tiled3<TX, TY, ZC, 0>), dim3(blocks), dim3(TX * TY), 0, st, g,
+                         ph, a);
+    else
+      hipLaunchKernelGGL((k_home_tiled3<TX, TY, ZC, 1>), dim3(blocks), dim3(TX * TY), 0, st, g,
+                         ph, a);
+  } else if constexpr (FORM == 2) {
     if (ph.bounce == 0)
       hipLaunchKernelGGL((k_home_tiled2<TX, TY, ZC, 0>), dim3(blocks), dim3(TX * TY), 0, st, g,
                          ph, a);
@@ -126,6 +147,8 @@ hipError_t launch_tiled(const Geo& g, const Phys& ph, const HomeLaunch& a, hipSt
 bool home_variant_ok(const Geo& g, int variant) {
   if (variant == kHomePlain) return true;
   if (variant < kHomePlain || variant >= kHomeVariants) return false;
+  // the third form addresses a moment plane with 32-bit byte offsets
+  if (variant >= kHome3T4Z16 && uint64_t(g.P) * sizeof(float) >= (uint64_t(1) << 32)) return false;
   // the tiled kernels take an outlet's moments from the lane at x-1, which
   // must sit in the same 64-wide tile: not so when nx-1 starts a tile
   return (g.nx - 1) % 64 != 0;
@@ -159,11 +182,14 @@ hipError_t launch_home_step(const Geo& g, const Phys& ph, const HomeLaunch& a, i
     case kHomeT4Z16: return launch_tiled<4, 16>(g, ph, a, st);
     case kHomeT8Z16: return launch_tiled<8, 16>(g, ph, a, st);
     case kHomeT8Z32: return launch_tiled<8, 32>(g, ph, a, st);
-    case kHome2T4Z16: return launch_tiled<4, 16, true>(g, ph, a, st);
-    case kHome2T8Z16: return launch_tiled<8, 16, true>(g, ph, a, st);
-    case kHome2T8Z32: return launch_tiled<8, 32, true>(g, ph, a, st);
-    case kHome2T16Z32: return launch_tiled<16, 32, true>(g, ph, a, st);
-    case kHome2T8Z64: return launch_tiled<8, 64, true>(g, ph, a, st);
+    case kHome2T4Z16: return launch_tiled<4, 16, 2>(g, ph, a, st);
+    case kHome2T8Z16: return launch_tiled<8, 16, 2>(g, ph, a, st);
+    case kHome2T8Z32: return launch_tiled<8, 32, 2>(g, ph, a, st);
+    case kHome2T16Z32: return launch_tiled<16, 32, 2>(g, ph, a, st);
+    case kHome2T8Z64: return launch_tiled<8, 64, 2>(g, ph, a, st);
+    case kHome3T4Z16: return launch_tiled<4, 16, 3>(g, ph, a, st);
+    case kHome3T8Z32: return launch_tiled<8, 32, 3>(g, ph, a, st);
+    case kHome3T8Z64: return launch_tiled<8, 64, 3>(g, ph, a, st);
     default: break;
   }
   if (ph.bounce == 0)
