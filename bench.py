@@ -121,7 +121,9 @@ def run_method(case, method, steps, warmup, band=0):
         s.set_inlet_profile(case.inlet)
     s.reinit_from_macro(*case.fields)
     variant = s.variant
-    pair = method != 1 and variant != 0
+    # two steps per launch: the BGK / Hybrid tiles (every variant but the AA
+    # kernel) and HOME's two-step tile (variant 12)
+    pair = variant != 0 if method != 1 else variant == 12
     if pair:
         steps, warmup = steps + steps % 2, warmup + warmup % 2
     if warmup:
@@ -131,11 +133,13 @@ def run_method(case, method, steps, warmup, band=0):
     s.close()
     per = ms / steps
     b_lup = B_LUP[method]
-    kernel_b = B_PAIR if pair else B_LUP[method]
+    kernel_b = (B_PAIR if method != 1 else B_LUP[method] / 2) if pair else B_LUP[method]
     gbps = case.fluid() * b_lup / (per * 1e-3) / 1e9
     steps_per_launch = 2 if pair else 1
     kernel = ("k_pop_pair" if pair else "k_pop_step") if method != 1 else (
-        "k_home_step" if variant == 0 else ("k_home_tiled" if variant <= 3 else "k_home_tiled2"))
+        "k_home_step" if variant == 0 else ("k_home_tiled" if variant <= 3 else (
+            "k_home_tiled2" if variant <= 8 else ("k_home_tiled3" if variant <= 11
+                                                  else "k_home_pair"))))
     return {"MLUPS": round(case.n * steps / (ms * 1e-3) / 1e6, 1),
             "ms_per_step": round(per, 5), "B_per_LUP": b_lup, "variant": variant,
             "kernel": kernel, "launch_ms": round(per * steps_per_launch, 5),

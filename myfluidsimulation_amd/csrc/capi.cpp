@@ -142,15 +142,28 @@ void free_solver(fs_solver* s) {
 
 
 int upload_flags(fs_solver* s) {
-  s->outlet_tile_start = s->outlet_tile_start32 = false;
-  for (int z = 0; z < s->g.nz; ++z)
-    for (int y = 0; y < s->g.ny; ++y) {
-      const uint8_t* row = s->hflags.data() + (size_t(z) * s->g.ny + y) * s->g.nx;
-      for (int x = 0; x < s->g.nx; x += 32)
+  s->outlet_tile_start = s->outlet_tile_start32 = s->outlet_ring_read = false;
+  const int nx = s->g.nx, ny = s->g.ny, nz = s->g.nz;
+  const uint8_t* fl = s->hflags.data();
+  auto at = [&](int x, int y, int z) {
+    return fl[(size_t((z + nz) % nz) * ny + size_t((y + ny) % ny)) * nx + size_t(x % nx)];
+  };
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y) {
+      const uint8_t* row = fl + (size_t(z) * ny + y) * nx;
+      for (int x = 0; x < nx; x += 32)
         if (row[x] & FS_FLAG_OUTLET) {
           s->outlet_tile_start32 = true;
           if (x % 64 == 0) s->outlet_tile_start = true;
         }
+      for (int x = 63; x < nx + 63; x += 64) {
+        const int xo = x < nx ? x : nx - 1;  // x % 64 == 63, then nx-1
+        if (!(row[xo] & FS_FLAG_OUTLET)) continue;
+        for (int dz = -1; dz <= 1; ++dz)
+          for (int dy = -1; dy <= 1; ++dy)
+            if (!(at(xo + 1, y + dy, z + dz) & (FS_FLAG_INLET | FS_FLAG_SOLID)))
+              s->outlet_ring_read = true;
+      }
     }
   s->f2_synced = false;
   FS_HIP(hipMemcpyAsync(s->stage, s->hflags.data(), s->g.n, hipMemcpyHostToDevice, s->stream));
@@ -243,7 +256,8 @@ static int solver_pop_variant(const fs_solver* s) {
 
 int solver_home_variant(const fs_solver* s) {
   if (s->outlet_tile_start) return fs::kHomePlain;
-  return s->home_variant >= 0 ? s->home_variant : fs::home_default_variant(s->g);
+  const int v = s->home_variant >= 0 ? s->home_variant : fs::home_default_variant(s->g);
+  return fs::home_variant_pair(v) && s->outlet_ring_read ? fs::home_single_variant(s->g, v) : v;
 }
 
 namespace {
@@ -252,7 +266,15 @@ int enqueue_steps(fs_solver* s, int nsteps) {
   if (nsteps < 0) return fail(FS_ERR_INVALID, "nsteps must be >= 0");
   if (s->method == fs::kMethodHOME) {
     const int variant = solver_home_variant(s);
-    for (int i = 0; i < nsteps; ++i) {
+    int i = 0;
+    if (fs::home_variant_pair(variant)) {
+      for (; i + 2 <= nsteps; i += 2) {
+        fs::HomeLaunch a{s->mbuf[s->cur], s->mbuf[1 - s->cur], s->flags, s->nbm, s->inlet};
+        FS_HIP(fs::launch_home_pair(s->g, s->ph, a, variant, s->stream));
+        s->cur = 1 - s->cur;
+      }
+    }
+    for (; i < nsteps; ++i) {
       fs::HomeLaunch a{s->mbuf[s->cur], s->mbuf[1 - s->cur], s->flags, s->nbm, s->inlet};
       FS_HIP(fs::launch_home_step(s->g, s->ph, a, variant, s->stream));
       s->cur = 1 - s->cur;

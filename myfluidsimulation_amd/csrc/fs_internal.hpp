@@ -165,14 +165,24 @@ struct HomeLaunch {
 };
 // HOME kernel variants: the plain per-cell gather, three tile shapes (64 x TY
 // columns, ZC-plane z-chunks) of the first tiled form and the same three of
-// the second (home_tile2), and three of the third (home_tile3)
+// the second (home_tile2), three of the third (home_tile3), and the two-step
+// tile (home_pair_tile: two steps per launch, 64 x 11 x 64)
 constexpr int kHomePlain = 0, kHomeT4Z16 = 1, kHomeT8Z16 = 2, kHomeT8Z32 = 3, kHome2T4Z16 = 4,
               kHome2T8Z16 = 5, kHome2T8Z32 = 6, kHome2T16Z32 = 7, kHome2T8Z64 = 8,
-              kHome3T4Z16 = 9, kHome3T8Z32 = 10, kHome3T8Z64 = 11, kHomeVariants = 12;
+              kHome3T4Z16 = 9, kHome3T8Z32 = 10, kHome3T8Z64 = 11, kHomeP11Z64 = 12,
+              kHomeVariants = 13;
 bool home_variant_ok(const Geo& g, int variant);
 int home_default_variant(const Geo& g);
-// variant < 0: the default for this grid
+// two steps per launch
+inline bool home_variant_pair(int variant) { return variant == kHomeP11Z64; }
+// the one-step variant that takes a single step for a two-step variant
+int home_single_variant(const Geo& g, int variant);
+// variant < 0: the default for this grid; a two-step variant takes one step
+// with its one-step counterpart
 hipError_t launch_home_step(const Geo& g, const Phys& ph, const HomeLaunch& a, int variant,
+                            hipStream_t st);
+// two steps m -> mn of a two-step variant
+hipError_t launch_home_pair(const Geo& g, const Phys& ph, const HomeLaunch& a, int variant,
                             hipStream_t st);
 hipError_t launch_home_fill(const Geo& g, float* m, float* mn, const float* rho, const float* ux,
                             const float* uy, const float* uz, hipStream_t st);

@@ -9,7 +9,7 @@
 // by default, the reference's own pairing table on request), and accumulates
 // the new moments (lbm3d_home.cu:238-285).  Per-direction arithmetic is specialised
 // at compile time (d3q19_dev.hpp) and keeps the reference's float order.
-#include "home_body.hpp"
+#include "home_pair.hpp"
 
 namespace fs {
 
@@ -94,6 +94,33 @@ __global__ void __launch_bounds__(TX * TY) k_home_tiled3(Geo g, Phys ph, HomeLau
   home_tile3<TX, TY, ZC, BB>(g, ph, a, bx, rest % tilesY, rest / tilesY, fs);
 }
 
+template <int TY, int ZC, int BB>
+__global__ void __launch_bounds__(64 * (TY + 5)) k_home_pair(Geo g, Phys ph, HomeLaunch a) {
+  __shared__ float lds[PairLds<TY>::FLOATS];
+  const int tilesX = (g.nx + 63) / 64;
+  // the XCD-contiguous tile order of k_home_tiled
+  uint32_t L0 = blockIdx.x;
+  const uint32_t total = gridDim.x;
+  if ((total & 7u) == 0) L0 = (L0 & 7u) * (total >> 3) + (L0 >> 3);
+  const int tilesY = (g.ny + TY - 1) / TY;
+  const int bx = int(L0 % uint32_t(tilesX));
+  const int rest = int(L0 / uint32_t(tilesX));
+  home_pair_tile<TY, ZC, BB>(g, ph, a, bx, rest % tilesY, rest / tilesY, lds);
+}
+
+template <int TY, int ZC>
+hipError_t launch_pair(const Geo& g, const Phys& ph, const HomeLaunch& a, hipStream_t st) {
+  const unsigned blocks =
+      unsigned(((g.nx + 63) / 64) * ((g.ny + TY - 1) / TY) * ((g.nz + ZC - 1) / ZC));
+  if (ph.bounce == 0)
+    hipLaunchKernelGGL((k_home_pair<TY, ZC, 0>), dim3(blocks), dim3(64 * (TY + 5)), 0, st, g, ph,
+                       a);
+  else
+    hipLaunchKernelGGL((k_home_pair<TY, ZC, 1>), dim3(blocks), dim3(64 * (TY + 5)), 0, st, g, ph,
+                       a);
+  return hipGetLastError();
+}
+
 // kern_reset_mom / kern_reinit_from_macro (lbm3d_home.cu:83-137): both buffers
 __global__ void __launch_bounds__(256) k_home_fill(Geo g, float* m, float* mn, const float* rho,
                                                    const float* ux, const float* uy,
@@ -147,7 +174,8 @@ hipError_t launch_tiled(const Geo& g, const Phys& ph, const HomeLaunch& a, hipSt
 bool home_variant_ok(const Geo& g, int variant) {
   if (variant == kHomePlain) return true;
   if (variant < kHomePlain || variant >= kHomeVariants) return false;
-  // the third form addresses a moment plane with 32-bit byte offsets
+  // the third form and the two-step tile address a moment plane with 32-bit
+  // byte offsets
   if (variant >= kHome3T4Z16 && uint64_t(g.P) * sizeof(float) >= (uint64_t(1) << 32)) return false;
   // the tiled kernels take an outlet's moments from the lane at x-1, which
   // must sit in the same 64-wide tile: not so when nx-1 starts a tile
@@ -161,23 +189,42 @@ int home_default_variant(const Geo& g) {
   // is faster
   const long tiles4 = long((g.nx + 63) / 64) * ((g.ny + 3) / 4) * ((g.nz + 15) / 16);
   if (tiles4 < 512) return kHomePlain;
-  // deep chunks cut the re-computed halo planes, but only pay when they
-  // still leave enough workgroups (>= 4 per CU) to fill the chip; measured
-  // (second form against the first): 512^3 52.3 vs 51.6 GLUPS, 256^3 46.7 vs
-  // 43.6, 768x256x256 channel 50.9 vs 48.1; 384x128x128 channel (64x4x16)
-  // 38.0 vs 35.0
+  // the third tiled form (fewer instructions, bounce-back values from its own
+  // emission) in the tile shapes the second form measured best in: deep
+  // chunks cut the re-computed halo planes but only pay while they still
+  // leave enough workgroups.  Measured (third form against the second, same
+  // shape): 512^3 (8,64) 57.0-58.1 against 55.7-56.4 GLUPS, 768x256x256
+  // channel (8,64) 53.1 against 51.6, 256^3 (8,64) 55.4-57.6 against
+  // 55.5-56.7, 384x128x128 channel (4,16) 43.2 against 43.0
   const long tiles8 = long((g.nx + 63) / 64) * ((g.ny + 7) / 8);
-  // 64-plane chunks halve the re-read planes again where the grid is deep
-  // and still gives two workgroups per CU: 512^3 53.6 against 53.1 GLUPS
-  // (32-plane chunks), 256^3 48.1 against 46.7; the 768x256x256 channel ties
-  if (g.nz >= 256 && tiles8 * ((g.nz + 63) / 64) >= 512) return kHome2T8Z64;
-  if (g.nz >= 128 && tiles8 * ((g.nz + 31) / 32) >= 1024) return kHome2T8Z32;
-  return kHome2T4Z16;
+  int v = kHome3T4Z16;
+  if (g.nz >= 256 && tiles8 * ((g.nz + 63) / 64) >= 512)
+    v = kHome3T8Z64;
+  else if (g.nz >= 128 && tiles8 * ((g.nz + 31) / 32) >= 1024)
+    v = kHome3T8Z32;
+  // 2^30 cells or more: the second form (64-bit addresses) in the same shape
+  if (!home_variant_ok(g, v))
+    v = v == kHome3T8Z64 ? kHome2T8Z64 : (v == kHome3T8Z32 ? kHome2T8Z32 : kHome2T4Z16);
+  return v;
+}
+
+int home_single_variant(const Geo& g, int variant) {
+  if (!home_variant_pair(variant)) return variant;
+  const int v = home_default_variant(g);
+  return home_variant_pair(v) ? kHome2T8Z32 : v;
+}
+
+hipError_t launch_home_pair(const Geo& g, const Phys& ph, const HomeLaunch& a, int variant,
+                            hipStream_t st) {
+  if (variant == kHomeP11Z64) return launch_pair<11, 64>(g, ph, a, st);
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_home_step(const Geo& g, const Phys& ph, const HomeLaunch& a, int variant,
                             hipStream_t st) {
   if (variant < 0 || !home_variant_ok(g, variant)) variant = home_default_variant(g);
+  variant = home_single_variant(g, variant);
+  if (!home_variant_ok(g, variant)) variant = kHomePlain;
   switch (variant) {
     case kHomeT4Z16: return launch_tiled<4, 16>(g, ph, a, st);
     case kHomeT8Z16: return launch_tiled<8, 16>(g, ph, a, st);

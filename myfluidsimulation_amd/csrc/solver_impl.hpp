@@ -44,6 +44,11 @@ struct fs_solver {
   // lane at x-1, do not apply
   bool outlet_tile_start = false;
   bool outlet_tile_start32 = false;
+  // an outlet cell sits on the left ring column of a 64-wide tile (x % 64 ==
+  // 63, or x = nx-1 for the first tile) and a cell that reads it across that
+  // column (x+1, y +-1, z +-1) is neither inlet nor solid: the two-step HOME
+  // tile, which cannot finish such an outlet, does not apply
+  bool outlet_ring_read = false;
   float* inlet = nullptr;  // ny*nz
   float* scratch = nullptr;  // 19 planes for state conversion (lazy)
 

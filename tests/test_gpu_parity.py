@@ -214,7 +214,7 @@ def test_home_tile_shapes(fs, oracle, dims):
     inlet = np.full(ny * nz, 0.04, np.float32)
     fields = wl.uniform_flow(nx * ny * nz, 0.04)
     tiled = (nx - 1) % 64 != 0
-    for variant in ((1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11) if tiled else (0,)):
+    for variant in (tuple(range(1, 13)) if tiled else (0,)):
         res = run_pair(fs, oracle, 1, (nx, ny, nz), 0.62, 9, flags=flags, fields=fields,
                        inlet=inlet, force=(1e-6, 0.0, 0.0), variant=variant)
         assert_exact(res, f"home tile {dims} variant {variant}")
@@ -238,7 +238,7 @@ def test_home_variants_and_autotune(fs, oracle):
     o.reinit(*fields)
     o.step(6)
     ref = o.state()
-    for variant in (0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, -1):
+    for variant in tuple(range(13)) + (-1,):
         with fs.Solver(1, nx, ny, nz, 0.58) as s:
             s.set_flags(flags)
             s.set_inlet_profile(inlet)
@@ -247,10 +247,47 @@ def test_home_variants_and_autotune(fs, oracle):
             if variant == -1:
                 before = s.state()
                 chosen = s.autotune(3)
-                assert chosen in range(12) and s.variant == chosen
+                assert chosen in range(13) and s.variant == chosen
                 np.testing.assert_array_equal(s.state(), before)
             s.step(6)
             np.testing.assert_array_equal(s.state(), ref)
+
+
+def test_home_pair_outlet_rules(fs, oracle):
+    """The two-step HOME tile (variant 12) where an outlet sits on a tile's
+    left ring column: a channel whose outlet column x = nx-1 (nx % 64 == 0)
+    wraps onto the inlet keeps the two-step kernel; an outlet at x = 63 whose
+    downstream cells at x = 64 are fluid sends the solver to a one-step
+    kernel.  Odd and even step counts, both bounce rules: the oracle's bits."""
+    nx, ny, nz = 128, 14, 10
+    obst = np.zeros((nz, ny, nx), np.uint8)
+    obst[3:6, 4:8, 30:36] = 1
+    flags = wl.channel_flags(nx, ny, nz, obst.reshape(-1))
+    assert (flags.reshape(nz, ny, nx)[1:-1, 1:-1, nx - 1] & 8).all()
+    inlet = np.linspace(0.03, 0.05, ny * nz).astype(np.float32)
+    fields = wl.uniform_flow(nx * ny * nz, 0.04)
+    mid = flags.copy().reshape(nz, ny, nx)
+    mid[4:7, 5:9, 63] = 8  # outlets inside the channel, fluid at x = 62 and 64
+    for fl, pair in ((flags, True), (mid.reshape(-1), False)):
+        for rule in (0, 1):
+            with fs.Solver(1, nx, ny, nz, 0.6) as s:
+                s.set_flags(fl)
+                s.set_inlet_profile(inlet)
+                s.set_bounce_rule(rule)
+                s.set_variant(12)
+                assert (s.variant == 12) == pair
+                s.reinit_from_macro(*fields)
+                o = oracle.Oracle(1, nx, ny, nz, 0.6)
+                o.set_flags(fl)
+                o.set_inlet(inlet)
+                o.set_bounce(rule)
+                o.reinit(*fields)
+                for k in (1, 4, 7):
+                    s.step(k)
+                    o.step(k)
+                    what = f"pair {pair} rule {rule} chunk {k}"
+                    np.testing.assert_array_equal(s.state(), o.state(), err_msg=what)
+                o.close()
 
 
 def pop_case(nx, ny, nz, seed=5):
@@ -483,13 +520,13 @@ def test_pop_random_geometries(fs, oracle, seed):
     o.close()
 
 
-@pytest.mark.parametrize("seed", range(24))
+@pytest.mark.parametrize("seed", range(26))
 def test_home_random_geometries(fs, oracle, seed):
     """Seeded random small domains through every HOME kernel variant, each
     with both bounce-back rules: the oracle's bits."""
     dims, flags, fields, inlet, _, _, chunks, force = random_pop_case(100 + seed)
     flags = flags & np.uint8(0xFD)  # HOME has no band
-    variant, rule = seed % 12, (seed // 12) % 2
+    variant, rule = seed % 13, (seed // 13) % 2
     o = oracle.Oracle(1, *dims, 0.61, force)
     o.set_flags(flags)
     o.set_inlet(inlet)
