@@ -106,7 +106,7 @@ def tgv_case(n, tau=0.6, u0=0.05):
     return Case(n, n, n, tau, None, wl.taylor_green_3d(n, n, n, u0))
 
 
-def run_method(case, method, steps, warmup, band=0):
+def run_method(case, method, steps, warmup, band=0, variant=None):
     from myfluidsimulation_amd import Solver
     from myfluidsimulation_amd import solver as S
 
@@ -120,6 +120,8 @@ def run_method(case, method, steps, warmup, band=0):
     if case.inlet is not None:
         s.set_inlet_profile(case.inlet)
     s.reinit_from_macro(*case.fields)
+    if variant is not None:
+        s.set_variant(variant)
     variant = s.variant
     # two steps per launch: the BGK / Hybrid tiles (every variant but the AA
     # kernel) and HOME's two-step tile (variant 12)
@@ -461,12 +463,12 @@ def roofline_of(r, traffic, ceiling):
             "measured_copy_ceiling_GBps": round(ceiling, 1)}
 
 
-def run_set(case, runs, steps, warmup, workload, tag=""):
+def run_set(case, runs, steps, warmup, workload, tag="", variant=None):
     """runs: (name, method, band) -> {name: result}, each result carrying the
     profiled traffic of its kernel variant on this workload (or None)"""
     out = {}
     for name, m, band in runs:
-        r = run_method(case, m, steps, warmup, band)
+        r = run_method(case, m, steps, warmup, band, variant)
         r["traffic"] = traffic_lookup(workload, NAMES[m], r["kernel"], r["variant"])
         out[name] = r
         print(f"# {tag}{name}: {r['MLUPS']:.1f} MLUPS, {r['ms_per_step'] * 1e3:.1f} us/step, "
@@ -515,6 +517,9 @@ def main():
     ap.add_argument("--methods", default=None,
                     help="single-workload runs: comma-separated subset of the workload's "
                          "methods (e.g. home), for profiling one kernel")
+    ap.add_argument("--variant", type=int, default=None,
+                    help="single-workload runs: kernel variant of every method run (as "
+                         "fs_solver_set_variant), for profiling or comparing one kernel")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-extras", action="store_true",
                     help="roofline workload: only the 512^3 and 768x256x256 runs (no 384x128x128 "
@@ -564,7 +569,7 @@ def main():
         steps = args.steps or dsteps
         warmup = dwarm if args.warmup is None else args.warmup
         case = build()
-        methods = run_set(case, runs, steps, warmup, args.workload)
+        methods = run_set(case, runs, steps, warmup, args.workload, variant=args.variant)
         head = methods[runs[0][0]]
         line = base_line(head["MLUPS"], steps, warmup, head["ms_per_step"],
                          desc + f"; value = {runs[0][0]}",

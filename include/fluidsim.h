@@ -200,7 +200,8 @@ typedef struct fs_domain_desc {
 } fs_domain_desc;
 typedef struct fs_batch fs_batch;
 
-/* Allocates and resets (rho=1, u=0) every domain. */
+/* Allocates and resets (rho=1, u=0) every domain.  A domain has fewer than
+ * 2^31 cells, a HOME domain fewer than 2^29 (FS_ERR_INVALID otherwise). */
 int fs_batch_create(const fs_domain_desc* descs, int ndomains, fs_batch** out);
 int fs_batch_destroy(fs_batch* b);
 int fs_batch_info(const fs_batch* b, int* ndomains, size_t* cells);

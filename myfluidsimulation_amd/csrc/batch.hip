@@ -12,8 +12,8 @@
 // domains advance two steps per launch (k_batch_pair: 64 x 4 tiles where nx
 // is a multiple of 64, else 32 x 8 tiles, which waste fewer lanes than
 // 64-wide rows on the sweep's other edges of 48..112 cells), or one
-// (k_batch_pop) for an odd step count.  HOME domains use the second tiled
-// form (home_tile2) on the same two tile shapes.
+// (k_batch_pop) for an odd step count.  HOME domains use the third tiled
+// form (home_tile3) on the same two tile shapes.
 #include "home_body.hpp"
 #include "pair_body.hpp"
 #include "pop_body.hpp"
@@ -125,16 +125,16 @@ __global__ void __launch_bounds__(kBatchHomeTX * kBatchHomeTY)
     const int bx = rest % D.tilesX, by = rest / D.tilesX;
     if (D.wide) {
       if (D.ph.bounce == 0)
-        home_tile2<kBatchHomeWideTX, kBatchHomeWideTY, kBatchHomeZC, 0>(D.g, D.ph, a, bx, by, bz,
+        home_tile3<kBatchHomeWideTX, kBatchHomeWideTY, kBatchHomeZC, 0>(D.g, D.ph, a, bx, by, bz,
                                                                          fs);
       else
-        home_tile2<kBatchHomeWideTX, kBatchHomeWideTY, kBatchHomeZC, 1>(D.g, D.ph, a, bx, by, bz,
+        home_tile3<kBatchHomeWideTX, kBatchHomeWideTY, kBatchHomeZC, 1>(D.g, D.ph, a, bx, by, bz,
                                                                          fs);
     } else {
       if (D.ph.bounce == 0)
-        home_tile2<kBatchHomeTX, kBatchHomeTY, kBatchHomeZC, 0>(D.g, D.ph, a, bx, by, bz, fs);
+        home_tile3<kBatchHomeTX, kBatchHomeTY, kBatchHomeZC, 0>(D.g, D.ph, a, bx, by, bz, fs);
       else
-        home_tile2<kBatchHomeTX, kBatchHomeTY, kBatchHomeZC, 1>(D.g, D.ph, a, bx, by, bz, fs);
+        home_tile3<kBatchHomeTX, kBatchHomeTY, kBatchHomeZC, 1>(D.g, D.ph, a, bx, by, bz, fs);
     }
   }
 }

@@ -242,7 +242,9 @@ int fs_batch_create(const fs_domain_desc* descs, int ndomains, fs_batch** out) {
       const fs_domain_desc& s = descs[d];
       if (s.method < 0 || s.method > 2) return bail(fail(FS_ERR_INVALID, "domain %d: method", d));
       if (s.nx < 2 || s.ny < 2 || s.nz < 1 ||
-          (unsigned long long)s.nx * s.ny * s.nz >= (1ull << 31))
+          (unsigned long long)s.nx * s.ny * s.nz >= (1ull << 31) ||
+          // HOME tiles address a moment plane with 32-bit byte offsets
+          (s.method == 1 && (unsigned long long)s.nx * s.ny * s.nz >= (1ull << 29)))
         return bail(fail(FS_ERR_INVALID, "domain %d: bad size", d));
       if (!(s.tau > 0.5f) || !std::isfinite(s.tau))
         return bail(fail(FS_ERR_INVALID, "domain %d: tau must be > 0.5", d));

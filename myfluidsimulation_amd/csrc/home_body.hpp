@@ -780,19 +780,20 @@ __device__ __forceinline__ void home_tile3(const Geo& g, const Phys& ph, const H
   auto step = [&](auto kc, int p) {
     constexpr int K = decltype(kc)::value;
     constexpr int S2 = K % 2, N2 = (K + 1) % 2;  // this plane's / the other slot of two
-    constexpr int S3 = K % 3, O3 = (K + 1) % 3;  // this plane's / two planes back, of three
+    // slots of three: this plane's, two planes back, one plane back
+    constexpr int S3 = K % 3, O3 = (K + 1) % 3, M3 = (K + 2) % 3;
     const Mom& cur = Mm[S2];
     const uint32_t pOut = uint32_t((p - 1 + g.nz) % g.nz);
     // the output cell of this plane step is (x, y, p-1); its flags were read
     // two planes ago; an inlet's velocity and the solid-neighbour mask of
     // this plane's cell are read here, ahead of the reconstruction
-    const uint8_t flO = Fl[(K + 2) % 3], flP = Fl[S3];
+    const uint8_t flO = Fl[M3], flP = Fl[S3];
     float uinFin = 0.0f;
     if (active && (flO & F_INLET)) uinFin = a.inlet[size_t(pOut) * g.ny + y];
     uint32_t nbCur = 0u;
     if (active && (flP & F_NEAR)) nbCur = a.nbm[uint32_t((p + g.nz) % g.nz) * g.nxy + cxy];
     if (p < last) load_plane(p + 1, Mm[N2], Hm[N2]);
-    if (p < last && active) Fl[(K + 1) % 3] = a.flags[uint32_t((p + 1 + g.nz) % g.nz) * g.nxy + cxy];
+    if (p < last && active) Fl[O3] = a.flags[uint32_t((p + 1 + g.nz) % g.nz) * g.nxy + cxy];
     float f0 = 0.0f, f5 = 0.0f, f6 = 0.0f;
     if (active) {
       const HomeSrc h(cur, omm);
@@ -972,7 +973,7 @@ __device__ __forceinline__ void home_tile3(const Geo& g, const Phys& ph, const H
     // own position; e_z = 0 and +1 directions go straight into the ring, the
     // e_z = -1 ones wait for the plane above
     if (__ballot(nbCur != 0u) != 0) {
-      float* A = Ar[(K + 2) % 3];  // e_z = +1 values of output p (read one plane ago)
+      float* A = Ar[M3];  // e_z = +1 values of output p (read one plane ago)
       float* B = Br[S2];
       for_q<1, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
