@@ -75,6 +75,25 @@ __device__ __forceinline__ float feq_q(const WRho& wr, float ux, float uy, float
   return wr.template get<Q>() * (((1.0f + 3.0f * eu) + (4.5f * eu) * eu) - uu15);
 }
 
+// feq_q<Q> and feq_q<OPP[Q]> together, with the same bits: e_opp = -e_q, so
+// the opposite direction's e.u is -eu (negation is exact), its 3*eu is
+// -(3*eu), its (4.5*eu)*eu the same product (round-to-nearest is symmetric),
+// and 1 + (-t) is 1 - t
+template <int Q>
+__device__ __forceinline__ void feq_pair(const WRho& wr, float ux, float uy, float uz, float uu15,
+                                         float& fq, float& fo) {
+  static_assert(Q != 0, "the rest direction has no partner");
+  const float eu = edot<Q>(ux, uy, uz);
+  const float t1 = 3.0f * eu;
+  const float t2 = (4.5f * eu) * eu;
+  const float w = wr.template get<Q>();
+  fq = w * (((1.0f + t1) + t2) - uu15);
+  fo = w * (((1.0f - t1) + t2) - uu15);
+}
+
+// The nine (q, OPP[q]) pairs of D3Q19 by their first member.
+constexpr int kPairFirst[9] = {1, 3, 5, 7, 8, 11, 12, 15, 16};
+
 // kern_reinit_eq's cs2 form (lbm_reinit.cu:31-49):
 //   w*r*(1 + cu/cs2 + 0.5*(cu*cu)/(cs2*cs2) - 0.5*uu/cs2)
 template <int Q>

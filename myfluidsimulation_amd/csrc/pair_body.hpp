@@ -138,20 +138,20 @@ __device__ __forceinline__ void pair_tile(const Geo& g, const Phys& ph, const Pa
     if (active && !(curFl & F_SOLID)) {
       if (!tileInletWave) {
         const PopCollision<HYB> pc(pop_sums(nf), ph, (curFl & F_BAND) != 0);
-        for_q<0, 19>([&](auto qc) {
-          constexpr int q = decltype(qc)::value;
-          lds[L::at(q, ly + 1, lx + 1)] = pc.template post<q>(nf[q]);
-        });
+        float post[19];
+        pc.post_all(nf, post);
+#pragma unroll
+        for (int q = 0; q < 19; ++q) lds[L::at(q, ly + 1, lx + 1)] = post[q];
       } else {
         float in[19];
 #pragma unroll
         for (int q = 0; q < 19; ++q) in[q] = tileInlet ? 0.0f : nf[q];
         PopCollision<HYB> pc(pop_sums(in), ph, (curFl & F_BAND) != 0);
         if (tileInlet) pc.set_inlet(curUin);
-        for_q<0, 19>([&](auto qc) {
-          constexpr int q = decltype(qc)::value;
-          lds[L::at(q, ly + 1, lx + 1)] = pc.template post<q>(in[q]);
-        });
+        float post[19];
+        pc.post_all(in, post);
+#pragma unroll
+        for (int q = 0; q < 19; ++q) lds[L::at(q, ly + 1, lx + 1)] = post[q];
       }
     }
     // ---- step t on plane p: ring cells ----
@@ -239,10 +239,10 @@ __device__ __forceinline__ void pair_tile(const Geo& g, const Phys& ph, const Pa
     }
     if (live && !anyInlet) {
       const PopCollision<HYB> pc(pop_sums(f), ph, (fl & F_BAND) != 0);
-      for_q<0, 19>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        f[q] = pc.template post<q>(f[q]);
-      });
+      float in[19];
+#pragma unroll
+      for (int q = 0; q < 19; ++q) in[q] = f[q];
+      pc.post_all(in, f);
       r = pc.r;
       u = pc.ux;
       v = pc.uy;
@@ -250,10 +250,10 @@ __device__ __forceinline__ void pair_tile(const Geo& g, const Phys& ph, const Pa
     } else if (live) {
       PopCollision<HYB> pc(pop_sums(f), ph, (fl & F_BAND) != 0);
       if (inl) pc.set_inlet(uinPrev);
-      for_q<0, 19>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        f[q] = pc.template post<q>(f[q]);
-      });
+      float in[19];
+#pragma unroll
+      for (int q = 0; q < 19; ++q) in[q] = f[q];
+      pc.post_all(in, f);
       r = pc.r;
       u = pc.ux;
       v = pc.uy;

@@ -51,12 +51,28 @@ struct PopCollision {
     osh = ph.omegaS - ph.omegaH;
   }
 
-  template <int Q>
-  __device__ __forceinline__ float post(float fq) const {
-    const float fe = feq_q<Q>(wr, ux, uy, uz, uu15);
+  __device__ __forceinline__ float relax(float fq, float fe) const {
     if (!HYB || band) return fq + om * (fe - fq);
     const float ds = fq - fe;
     return (fq + oh * (fe - fq)) + osh * (-ds);
+  }
+
+  template <int Q>
+  __device__ __forceinline__ float post(float fq) const {
+    return relax(fq, feq_q<Q>(wr, ux, uy, uz, uu15));
+  }
+
+  // out[q] = post<q>(in[q]) for all 19 directions, each direction's
+  // equilibrium computed together with its opposite's (feq_pair: same bits)
+  __device__ __forceinline__ void post_all(const float (&in)[19], float (&out)[19]) const {
+    out[0] = post<0>(in[0]);
+    for_q<0, 9>([&](auto kc) {
+      constexpr int q = kPairFirst[decltype(kc)::value], o = OPP[q];
+      float eq, eo;
+      feq_pair<q>(wr, ux, uy, uz, uu15, eq, eo);
+      out[q] = relax(in[q], eq);
+      out[o] = relax(in[o], eo);
+    });
   }
 
   // An inlet cell emits feq(1, (u_in, 0, 0)) whatever streams into it
@@ -81,10 +97,9 @@ template <bool HYB>
 __device__ __forceinline__ void collide(float (&f)[19], float& r, float& ux, float& uy,
                                         float& uz, const Phys& ph, bool band) {
   const PopCollision<HYB> pc(pop_sums(f), ph, band);
-  for_q<0, 19>([&](auto qc) {
-    constexpr int q = decltype(qc)::value;
-    f[q] = pc.template post<q>(f[q]);
-  });
+  const float in[19] = {f[0], f[1], f[2],  f[3],  f[4],  f[5],  f[6],  f[7],  f[8], f[9],
+                        f[10], f[11], f[12], f[13], f[14], f[15], f[16], f[17], f[18]};
+  pc.post_all(in, f);
   r = pc.r;
   ux = pc.ux;
   uy = pc.uy;
