@@ -378,6 +378,28 @@ __device__ __forceinline__ float emit_corner(const HomeSrc& h, float sux, float 
   return feq + h.omm * (w45 * contraction);
 }
 
+// emit<Q>'s arithmetic for a direction whose y component is sign sy (a
+// lane value), as emit_sx for x: (CX, sy, CZ) with at most one of CX, CZ
+// nonzero; suy, ssxy, ssyz are u_y, S_xy, S_yz times sy.
+template <int CX, int CZ>
+__device__ __forceinline__ float emit_sy(const HomeSrc& h, float suy, float ssxy, float ssyz) {
+  constexpr float w = (CX == 0 && CZ == 0) ? 1.0f / 18.0f : 1.0f / 36.0f;
+  constexpr float w45 = 4.5f * w;
+  const float wr = (CX == 0 && CZ == 0) ? h.wr1 : h.wr7;
+  float eu;
+  if constexpr (CX == 0 && CZ == 0)
+    eu = suy;
+  else if constexpr (CZ == 0)
+    eu = sg<CX>(h.ux) + suy;
+  else
+    eu = suy + sg<CZ>(h.uz);
+  const float feq = wr * (((1.0f + 3.0f * eu) + (4.5f * eu) * eu) - h.uu15);
+  float contraction = ((CX ? h.pA[0] : h.pB[0]) + h.pA[1]) + (CZ ? h.pA[2] : h.pB[2]);
+  if constexpr (CX != 0) contraction += 2.0f * sg<CX>(ssxy);
+  if constexpr (CZ != 0) contraction += 2.0f * sg<CZ>(ssyz);
+  return feq + h.omm * (w45 * contraction);
+}
+
 template <int TX, int TY, int ZC, int BB>
 __device__ __forceinline__ void home_tile2(const Geo& g, const Phys& ph, const HomeLaunch& a,
                                            int bx, int by, int bz, float* fs) {
@@ -833,18 +855,19 @@ __device__ __forceinline__ void home_tile3(const Geo& g, const Phys& ph, const H
     if (role >= 0) {
       const HomeSrc h(Hm[S2], omm);
       const int row = hy + 1, col = hx + 1;
-      if (role == 0) {  // e_y = +1: q = 3, 7, 8, 15, 17
-        fs[L::at(kLds16[3], row, col)] = h.template emit<3>();
-        fs[L::at(kLds16[7], row, col)] = h.template emit<7>();
-        fs[L::at(kLds16[8], row, col)] = h.template emit<8>();
-        fs[L::at(kLds16[15], row, col)] = h.template emit<15>();
-        fs[L::at(kLds16[17], row, col)] = h.template emit<17>();
-      } else if (role == 1) {  // e_y = -1: q = 4, 9, 10, 16, 18
-        fs[L::at(kLds16[4], row, col)] = h.template emit<4>();
-        fs[L::at(kLds16[9], row, col)] = h.template emit<9>();
-        fs[L::at(kLds16[10], row, col)] = h.template emit<10>();
-        fs[L::at(kLds16[16], row, col)] = h.template emit<16>();
-        fs[L::at(kLds16[18], row, col)] = h.template emit<18>();
+      if (role <= 1) {
+        // the row below (e_y = +1: q = 3, 7, 8, 15, 17) or above (e_y = -1:
+        // q = 4, 9, 10, 16, 18): one code path, the sign of e_y per lane; the
+        // e_y = -1 planes follow the e_y = +1 ones (kLds16)
+        const bool pos = role == 0;
+        const float suy = pos ? h.uy : -h.uy, ssxy = pos ? h.sxy : -h.sxy,
+                    ssyz = pos ? h.syz : -h.syz;
+        const int n = pos ? 0 : 1;
+        fs[L::at(2 + n, row, col)] = emit_sy<0, 0>(h, suy, ssxy, ssyz);
+        fs[L::at(4 + 2 * n, row, col)] = emit_sy<1, 0>(h, suy, ssxy, ssyz);
+        fs[L::at(5 + 2 * n, row, col)] = emit_sy<-1, 0>(h, suy, ssxy, ssyz);
+        fs[L::at(12 + n, row, col)] = emit_sy<0, 1>(h, suy, ssxy, ssyz);
+        fs[L::at(14 + n, row, col)] = emit_sy<0, -1>(h, suy, ssxy, ssyz);
       } else if (role == 2) {  // e_x = hsgn: (s,0,0) (s,1,0) (s,-1,0) (s,0,1) (s,0,-1)
         const bool pos = hsgn > 0;
         const float sux = pos ? h.ux : -h.ux, ssxy = pos ? h.sxy : -h.sxy,
@@ -924,48 +947,37 @@ __device__ __forceinline__ void home_tile3(const Geo& g, const Phys& ph, const H
       });
       home_finish(r, jx, jy, jz, Mxx, Mxy, Mxz, Myy, Myz, Mzz, ph, out);
     };
-    const bool plain = fin && (flO & (F_SOLID | F_INLET | F_OUTLET)) == 0;
-    if (__ballot(plain) == ~0ull) {
-      // the whole row is fluid (near-wall values already in place)
-      float f[19], out[10];
+    // every lane sums its gathered values (one copy of the sums in the
+    // unrolled body); solid, inlet and outlet cells then replace the result
+    float out[10];
+    {
+      float f[19];
       gather(f);
       moments(f, out);
-      const uint32_t co4 = pOut * plane4 + cxy4;
+    }
+    const uint8_t fl = fin ? flO : uint8_t(0);
+    if (__ballot((fl & (F_SOLID | F_INLET | F_OUTLET)) != 0) != 0) {
+      if (fl & (F_SOLID | F_INLET)) {
 #pragma unroll
-      for (int k = 0; k < 10; ++k) st_b(a.mn + size_t(k) * P, co4, out[k]);
-    } else {
-      const uint32_t co4 = pOut * plane4 + (fin ? cxy4 : 0u);
-      const uint8_t fl = fin ? flO : uint8_t(0);
-      const bool outlet = fin && (fl & F_OUTLET) != 0;
-      const bool anyOutlet = __ballot(outlet) != 0;
-      float out[10];
-#pragma unroll
-      for (int k = 0; k < 10; ++k) out[k] = 0.0f;
-      if (fin) {
-        if (fl & F_SOLID) {
-          out[0] = 1.0f;
-        } else if (fl & F_INLET) {
-          out[0] = 1.0f;
-          out[1] = uinFin;
-        } else {
-          float f[19];
-          gather(f);
-          moments(f, out);
-        }
+        for (int k = 0; k < 10; ++k) out[k] = 0.0f;
+        out[0] = 1.0f;
+        if (fl & F_INLET) out[1] = uinFin;
       }
       // zero-gradient outlet: the new moments of x-1, the previous lane (an
       // outlet never starts a tile row: the launchers check it)
-      if (anyOutlet) {
+      const bool outlet = (fl & F_OUTLET) != 0;
+      if (__ballot(outlet) != 0) {
 #pragma unroll
         for (int k = 0; k < 10; ++k) {
           const float left = __shfl_up(out[k], 1);
           if (outlet) out[k] = left;
         }
       }
-      if (fin) {
+    }
+    if (fin) {
+      const uint32_t co4 = pOut * plane4 + cxy4;
 #pragma unroll
-        for (int k = 0; k < 10; ++k) st_b(a.mn + size_t(k) * P, co4, out[k]);
-      }
+      for (int k = 0; k < 10; ++k) st_b(a.mn + size_t(k) * P, co4, out[k]);
     }
     // the next outputs of a near-wall cell of this plane: where c - e_q is
     // solid the cell takes its own post-collision value in direction OPP[q]

@@ -190,21 +190,17 @@ int home_default_variant(const Geo& g) {
   const long tiles4 = long((g.nx + 63) / 64) * ((g.ny + 3) / 4) * ((g.nz + 15) / 16);
   if (tiles4 < 512) return kHomePlain;
   // the third tiled form (fewer instructions, bounce-back values from its own
-  // emission) in the tile shapes the second form measured best in: deep
-  // chunks cut the re-computed halo planes but only pay while they still
-  // leave enough workgroups.  Measured (third form against the second, same
-  // shape): 512^3 (8,64) 57.0-58.1 against 55.7-56.4 GLUPS, 768x256x256
-  // channel (8,64) 53.1 against 51.6, 256^3 (8,64) 55.4-57.6 against
-  // 55.5-56.7, 384x128x128 channel (4,16) 43.2 against 43.0
+  // emission) with 32-plane chunks where they still leave >= 1024 workgroups
+  // (measured against 64-plane chunks, three runs each: 512^3 57.4-57.9
+  // against 57.4-58.1 GLUPS, 768x256x256 channel 53.0-54.4 against
+  // 53.0-53.2, 256^3 56.3-58.0 against 55.4-57.6), else 64x4x16 tiles
+  // (384x128x128 channel 46.8-47.6 GLUPS against 42.7-43.2 for the second
+  // form)
   const long tiles8 = long((g.nx + 63) / 64) * ((g.ny + 7) / 8);
   int v = kHome3T4Z16;
-  if (g.nz >= 256 && tiles8 * ((g.nz + 63) / 64) >= 512)
-    v = kHome3T8Z64;
-  else if (g.nz >= 128 && tiles8 * ((g.nz + 31) / 32) >= 1024)
-    v = kHome3T8Z32;
+  if (g.nz >= 128 && tiles8 * ((g.nz + 31) / 32) >= 1024) v = kHome3T8Z32;
   // 2^30 cells or more: the second form (64-bit addresses) in the same shape
-  if (!home_variant_ok(g, v))
-    v = v == kHome3T8Z64 ? kHome2T8Z64 : (v == kHome3T8Z32 ? kHome2T8Z32 : kHome2T4Z16);
+  if (!home_variant_ok(g, v)) v = v == kHome3T8Z32 ? kHome2T8Z32 : kHome2T4Z16;
   return v;
 }
 
