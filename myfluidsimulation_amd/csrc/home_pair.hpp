@@ -246,10 +246,12 @@ __device__ __forceinline__ void home_pair_tile(const Geo& g, const Phys& ph, con
   // finish output plane pOut now, the e_z = 0 / +1 ones go to the ring.
   // nbOut / nbCur: solid-neighbour masks of planes pOut and pOut+1 (the
   // plane just emitted); r0, r5, r6: this lane's own column values of the
-  // plane just emitted.  Returns the gathered values of pOut in f.
+  // plane just emitted.  Returns the q-ordered sums of pOut's gathered
+  // values (home_finish) in out, computed before the ring slots they come
+  // from are refilled.
   auto gather_stage = [&](auto wc, auto psc, PairRing& R, const float* Lx, int pos,
                           bool anyNearOut, bool anyNearCur, uint32_t nbOut, uint32_t nbCur,
-                          float r0, float r5, float r6, float (&f)[19]) {
+                          float r0, float r5, float r6, float (&out)[10]) {
     constexpr int W = decltype(wc)::value;
     constexpr int PS = decltype(psc)::value;
     auto lds_in = [&](auto qc) {  // value arriving along q from c - e_q (in-plane part)
@@ -276,6 +278,7 @@ __device__ __forceinline__ void home_pair_tile(const Geo& g, const Phys& ph, con
     };
     // output pOut: e_z = +1 from the ring (two planes back), e_z = 0 from
     // the ring (one plane back), e_z = -1 from this plane
+    float f[19];
     for_q<0, 19>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
       if constexpr (CZ[q] > 0) f[q] = R.Ao[kSlot[q]];
@@ -289,6 +292,7 @@ __device__ __forceinline__ void home_pair_tile(const Geo& g, const Phys& ph, con
           if ((nbOut >> OPP[q]) & 1u) f[q] = R.Cb[kSlot[q]];
       });
     }
+    moments(f, out);
     // ring: this plane's e_z = 0 values (next output), e_z = +1 (the one
     // after); the +1 values of the next output move up
 #pragma unroll
@@ -315,26 +319,19 @@ __device__ __forceinline__ void home_pair_tile(const Geo& g, const Phys& ph, con
     }
   };
 
-  // the step rules of an output cell: solid, inlet, outlet (x-1's new
-  // moments, the previous lane), fluid
-  auto finish_cell = [&](bool fin, uint8_t fl, float uin, const float (&f)[19],
-                         float (&out)[10]) {
-    const bool plain = !fin || (fl & (F_SOLID | F_INLET | F_OUTLET)) == 0;
-    if (__ballot(plain) == ~0ull) {
-      moments(f, out);
-      return;
-    }
+  // the step rules of an output cell on top of its gathered sums: solid
+  // (1, 0...), inlet (1, u_in, 0...), outlet (x-1's new moments, the
+  // previous lane)
+  auto finish_cell = [&](bool fin, uint8_t fl, float uin, float (&out)[10]) {
+    const uint8_t f2 = fin ? fl : uint8_t(0);
+    if (__ballot((f2 & (F_SOLID | F_INLET | F_OUTLET)) != 0) == 0) return;
+    if (f2 & (F_SOLID | F_INLET)) {
 #pragma unroll
-    for (int k = 0; k < 10; ++k) out[k] = 0.0f;
-    const bool outlet = fin && (fl & F_OUTLET) != 0;
-    if (fin && (fl & F_SOLID)) {
+      for (int k = 0; k < 10; ++k) out[k] = 0.0f;
       out[0] = 1.0f;
-    } else if (fin && (fl & F_INLET)) {
-      out[0] = 1.0f;
-      out[1] = uin;
-    } else {
-      moments(f, out);
+      if (f2 & F_INLET) out[1] = uin;
     }
+    const bool outlet = (f2 & F_OUTLET) != 0;
     if (__ballot(outlet) != 0) {
 #pragma unroll
       for (int k = 0; k < 10; ++k) {
@@ -377,18 +374,14 @@ __device__ __forceinline__ void home_pair_tile(const Geo& g, const Phys& ph, con
     const bool anyNear1 = __ballot(isE && (fl1 & F_NEAR)) != 0;
     const bool anyNearP = __ballot(isE && (flP & F_NEAR)) != 0;
     float m1[10];
-    {
-      float f[19];
-      if (isE) {
-        gather_stage(std::integral_constant<int, PL::WA>{},
-                     std::integral_constant<int, PL::PA>{}, RB, LA, posA, anyNear1, anyNearP,
-                     nb1, nb0, a0, a5, a6, f);
-      } else {
+    if (isE) {
+      gather_stage(std::integral_constant<int, PL::WA>{}, std::integral_constant<int, PL::PA>{},
+                   RB, LA, posA, anyNear1, anyNearP, nb1, nb0, a0, a5, a6, m1);
+    } else {
 #pragma unroll
-        for (int q = 0; q < 19; ++q) f[q] = 0.0f;
-      }
-      finish_cell(finB, fl1, uin1, f, m1);
+      for (int k = 0; k < 10; ++k) m1[k] = 0.0f;
     }
+    finish_cell(finB, fl1, uin1, m1);
 
     // ---- C: emit m^{t+1}(o1) ----
     float c0 = 0.0f, c5 = 0.0f, c6 = 0.0f;
@@ -409,16 +402,15 @@ __device__ __forceinline__ void home_pair_tile(const Geo& g, const Phys& ph, con
       const bool finD = isT && o2 >= z0 && o2 < z0 + ZCe;
       const bool anyNear2 = __ballot(isT && (fl2 & F_NEAR)) != 0;
       const bool anyNearC = __ballot(isT && (fl1 & F_NEAR)) != 0;
-      float f[19], out[10];
+      float out[10];
       if (isT) {
-        gather_stage(std::integral_constant<int, PL::WC>{},
-                     std::integral_constant<int, PL::PC>{}, RD, LC, posC, anyNear2, anyNearC,
-                     nb2, nb1, c0, c5, c6, f);
+        gather_stage(std::integral_constant<int, PL::WC>{}, std::integral_constant<int, PL::PC>{},
+                     RD, LC, posC, anyNear2, anyNearC, nb2, nb1, c0, c5, c6, out);
       } else {
 #pragma unroll
-        for (int q = 0; q < 19; ++q) f[q] = 0.0f;
+        for (int k = 0; k < 10; ++k) out[k] = 0.0f;
       }
-      finish_cell(finD, fl2, uin2, f, out);
+      finish_cell(finD, fl2, uin2, out);
       if (finD) {
         const uint32_t off = plane_of(o2) * plane4 + cxy4;
 #pragma unroll

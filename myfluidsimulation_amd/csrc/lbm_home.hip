@@ -182,12 +182,17 @@ bool home_variant_ok(const Geo& g, int variant) {
   return (g.nx - 1) % 64 != 0;
 }
 
-int home_default_variant(const Geo& g) {
+namespace {
+
+// The one-step default: the plain gather where the grid is too small for the
+// tiles, else the third tiled form.
+int home_one_step_default(const Geo& g) {
   if (!home_variant_ok(g, kHome2T8Z32)) return kHomePlain;
   // a workgroup walks its whole z-chunk: below 512 tiles of 64x4x16 (about
   // 128^3, measured) too few of them fill the chip and the per-cell gather
   // is faster
-  const long tiles4 = long((g.nx + 63) / 64) * ((g.ny + 3) / 4) * ((g.nz + 15) / 16);
+  const long cols = (g.nx + 63) / 64;
+  const long tiles4 = cols * ((g.ny + 3) / 4) * ((g.nz + 15) / 16);
   if (tiles4 < 512) return kHomePlain;
   // the third tiled form (fewer instructions, bounce-back values from its own
   // emission) with 32-plane chunks where they still leave >= 1024 workgroups
@@ -196,7 +201,7 @@ int home_default_variant(const Geo& g) {
   // 53.0-53.2, 256^3 56.3-58.0 against 55.4-57.6), else 64x4x16 tiles
   // (384x128x128 channel 46.8-47.6 GLUPS against 42.7-43.2 for the second
   // form)
-  const long tiles8 = long((g.nx + 63) / 64) * ((g.ny + 7) / 8);
+  const long tiles8 = cols * ((g.ny + 7) / 8);
   int v = kHome3T4Z16;
   if (g.nz >= 128 && tiles8 * ((g.nz + 31) / 32) >= 1024) v = kHome3T8Z32;
   // 2^30 cells or more: the second form (64-bit addresses) in the same shape
@@ -204,15 +209,35 @@ int home_default_variant(const Geo& g) {
   return v;
 }
 
+}  // namespace
+
+int home_default_variant(const Geo& g) {
+  const int v = home_one_step_default(g);
+  if (v == kHomePlain || !home_variant_ok(g, kHomeP11Z64)) return v;
+  // The two-step tile (1024-thread workgroups, one per CU) where it gives
+  // enough of them: 64x11 columns with 64-plane chunks from 2048 workgroups
+  // (512^3: 3008; 61.8 GLUPS against 57.7 for the third form), else 32-plane
+  // chunks from 2048 (768x256x256 channel: 2304; 56.7 against 53.6).  256^3
+  // (768 workgroups of 64x11x32) ties with the third form (57.8 against
+  // 57.5) and keeps it.
+  const long tiles11 = long((g.nx + 63) / 64) * ((g.ny + 10) / 11);
+  if (tiles11 * ((g.nz + 63) / 64) >= 2048) return kHomeP11Z64;
+  if (tiles11 * ((g.nz + 31) / 32) >= 2048) return kHomeP11Z32;
+  return v;
+}
+
 int home_single_variant(const Geo& g, int variant) {
-  if (!home_variant_pair(variant)) return variant;
-  const int v = home_default_variant(g);
-  return home_variant_pair(v) ? kHome2T8Z32 : v;
+  return home_variant_pair(variant) ? home_one_step_default(g) : variant;
 }
 
 hipError_t launch_home_pair(const Geo& g, const Phys& ph, const HomeLaunch& a, int variant,
                             hipStream_t st) {
-  if (variant == kHomeP11Z64) return launch_pair<11, 64>(g, ph, a, st);
+  switch (variant) {
+    case kHomeP11Z64: return launch_pair<11, 64>(g, ph, a, st);
+    case kHomeP11Z32: return launch_pair<11, 32>(g, ph, a, st);
+    case kHomeP10Z64: return launch_pair<10, 64>(g, ph, a, st);
+    default: break;
+  }
   return hipErrorInvalidValue;
 }
 

@@ -214,7 +214,7 @@ def test_home_tile_shapes(fs, oracle, dims):
     inlet = np.full(ny * nz, 0.04, np.float32)
     fields = wl.uniform_flow(nx * ny * nz, 0.04)
     tiled = (nx - 1) % 64 != 0
-    for variant in (tuple(range(1, 13)) if tiled else (0,)):
+    for variant in (tuple(range(1, 15)) if tiled else (0,)):
         res = run_pair(fs, oracle, 1, (nx, ny, nz), 0.62, 9, flags=flags, fields=fields,
                        inlet=inlet, force=(1e-6, 0.0, 0.0), variant=variant)
         assert_exact(res, f"home tile {dims} variant {variant}")
@@ -238,7 +238,7 @@ def test_home_variants_and_autotune(fs, oracle):
     o.reinit(*fields)
     o.step(6)
     ref = o.state()
-    for variant in tuple(range(13)) + (-1,):
+    for variant in tuple(range(15)) + (-1,):
         with fs.Solver(1, nx, ny, nz, 0.58) as s:
             s.set_flags(flags)
             s.set_inlet_profile(inlet)
@@ -247,7 +247,7 @@ def test_home_variants_and_autotune(fs, oracle):
             if variant == -1:
                 before = s.state()
                 chosen = s.autotune(3)
-                assert chosen in range(13) and s.variant == chosen
+                assert chosen in range(15) and s.variant == chosen
                 np.testing.assert_array_equal(s.state(), before)
             s.step(6)
             np.testing.assert_array_equal(s.state(), ref)
@@ -520,13 +520,13 @@ def test_pop_random_geometries(fs, oracle, seed):
     o.close()
 
 
-@pytest.mark.parametrize("seed", range(26))
+@pytest.mark.parametrize("seed", range(30))
 def test_home_random_geometries(fs, oracle, seed):
     """Seeded random small domains through every HOME kernel variant, each
     with both bounce-back rules: the oracle's bits."""
     dims, flags, fields, inlet, _, _, chunks, force = random_pop_case(100 + seed)
     flags = flags & np.uint8(0xFD)  # HOME has no band
-    variant, rule = seed % 13, (seed // 13) % 2
+    variant, rule = seed % 15, (seed // 15) % 2
     o = oracle.Oracle(1, *dims, 0.61, force)
     o.set_flags(flags)
     o.set_inlet(inlet)
