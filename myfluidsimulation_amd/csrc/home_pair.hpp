@@ -11,15 +11,17 @@
 // (66 x (TY+2) cells per plane), and m^{t+1} on E gathers what m^t emits on
 // F = E plus another ring (68 x (TY+4)), of which the outer ring only emits
 // the directions that point into E.  One lane per cell of E or of F's outer
-// ring (64 x (TY+5) lanes):
-//   waves 0 .. TY+1   rows y0-1 .. y0+TY of E (x0 .. x0+63); rows 1 .. TY are T
-//   wave  TY+2        E's two side columns (x0-1, x0+64), F's two side
+// ring (64 x (TY+5) lanes, one role per wave; pair_wave_role deals the roles
+// to the hardware waves so that the four SIMDs carry similar work):
+//   roles 0 .. TY+1   rows y0-1 .. y0+TY of E (x0 .. x0+63); rows 1 .. TY are T
+//   role  TY+2        E's two side columns (x0-1, x0+64), F's two side
 //                     columns (x0-2, x0+65), the four ends of F's rows and
-//                     F's four corners
-//   waves TY+3, TY+4  F's rows y0-2 and y0+TY+1
+//                     F's four corners (the side wave)
+//   roles TY+3, TY+4  F's rows y0-2 and y0+TY+1
 // Each lane walks the source planes p = z0-2 .. z0+ZC+1; per plane:
-//   A  every lane emits m^t(p) of its cell into LDS plane set A (E cells all
-//      19 directions, ring cells the five or one that enter E);  barrier;
+//   A  every lane emits m^t(p) of its cell into LDS plane set A (E cells and
+//      the side wave all 19 directions, F's rows the five that enter E);
+//      barrier;
 //   B  every E lane gathers m^{t+1}(p-1) (values from planes p-2, p-1, p,
 //      the first two kept in registers) in the reference's q order;
 //   C  E lanes emit m^{t+1}(p-1) into LDS plane set C (tile cells all
@@ -121,6 +123,28 @@ __device__ __forceinline__ void emit_all(float* L, int pos, const HomeSrc& h, fl
   h.template emit_pair<5>(r5, r6);
 }
 
+// The role (the wave number of the layout above) of hardware wave hw.  A
+// workgroup's waves are dealt to the CU's four SIMDs in turn (wave hw runs on
+// SIMD hw % 4), and every stage ends at a barrier, so a stage lasts as long
+// as its busiest SIMD.  With TY = 11 the sixteen roles are eleven tile rows
+// (stages A to D), E's two ring rows (A, B and five directions of C), the
+// side wave (A, B, ten directions of C) and F's two rows (five directions of
+// A).  In role order the side wave would join three tile rows on SIMD 1;
+// this order gives SIMD 0 the side wave, one F row and two tile rows, and the
+// others three tile rows and one E ring row or F row each (measured on the
+// 512^3 vortex: 68.7 -> 71.8 GLUPS).
+template <int TY>
+__device__ __forceinline__ int pair_wave_role(int hw) {
+  if constexpr (TY == 11) {
+    // hw:      0   1  2  3   4  5  6  7  8  9  10  11 12 13  14  15
+    // role:   13   1  2  3  14  5  6  7  8  9  10  11  4  0  12  15
+    constexpr uint64_t kRole = 0xFC04BA98765E321DULL;
+    return int((kRole >> (4 * hw)) & 15u);
+  } else {
+    return hw;
+  }
+}
+
 // Gather rings of one stage (B or D) for one lane: values with e_z = +1 (Ao
 // for the output due now, Am for the next one), e_z = 0 for the next output
 // (Br), and the own values that replace e_z = -1 values of the next output of
@@ -137,7 +161,8 @@ __device__ __forceinline__ void home_pair_tile(const Geo& g, const Phys& ph, con
   using PL = PairLds<TY>;
   float* LA = lds;
   float* LC = lds + 16 * PL::PA;
-  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+  const int tid = threadIdx.x, l = tid & 63;
+  const int w = pair_wave_role<TY>(tid >> 6);
   const int x0 = bx * TX, y0 = by * TY, z0 = bz * ZC;
   const int TXe = min(TX, g.nx - x0), TYe = min(TY, g.ny - y0), ZCe = min(ZC, g.nz - z0);
   const float omm = 1.0f - ph.omegaL;
@@ -359,10 +384,15 @@ __device__ __forceinline__ void home_pair_tile(const Geo& g, const Phys& ph, con
     }
 
     // ---- A: emit m^t(p) ----
+    // The side wave's lanes all emit every direction: its cells need nine
+    // different direction sets, which as divergent branches would cost it
+    // 43 directions instead of 19 (the extra values land in LDS slots that
+    // no gather reads).  Its SIMD sets the pace of this stage: 512^3 vortex
+    // 62.5 -> 68.7 GLUPS.
     float a0 = 0.0f, a5 = 0.0f, a6 = 0.0f;
     if (hasCell) {
       const HomeSrc h(Mm[K], omm);
-      if (setA == 0)
+      if (setA == 0 || w == TY + 2)
         emit_all<PL::PA>(LA, posA, h, a0, a5, a6);
       else
         emit_ring_set<PL::PA>(LA, posA, h, setA);
