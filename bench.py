@@ -124,7 +124,7 @@ def run_method(case, method, steps, warmup, band=0, variant=None):
         s.set_variant(variant)
     variant = s.variant
     # two steps per launch: the BGK / Hybrid tiles (every variant but the AA
-    # kernel) and HOME's two-step tiles (variants 12-14)
+    # kernel) and HOME's two-step tiles (variants 12-16)
     pair = variant != 0 if method != 1 else variant >= 12
     if pair:
         steps, warmup = steps + steps % 2, warmup + warmup % 2

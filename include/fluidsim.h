@@ -152,9 +152,10 @@ int fs_solver_timed_steps(fs_solver* s, int nsteps, float* elapsed_ms);
  * 3: 64x8 x 32; 4, 5, 6: the same shapes in a second form with fewer
  * instructions per update; 7: 64x16 x 32 and 8: 64x8 x 64 in that form;
  * 9: 64x4 x 16, 10: 64x8 x 32, 11: 64x8 x 64 in a third form; 12: 64x11 x
- * 64, 13: 64x11 x 32, 14: 64x10 x 64 tiles that advance two steps per launch,
- * the default on the largest grids, which run the one-step default where an
- * outlet's downstream neighbour reads it across a tile's left ring column).
+ * 64, 13: 64x11 x 32, 14: 64x10 x 64, 15: 64x12 x 64, 16: 64x12 x 32 tiles
+ * that advance two steps per launch, the default from about 256^3 cells,
+ * which run the one-step default where an outlet's downstream neighbour reads
+ * it across a tile's left ring column).
  * BGK / Hybrid have a one-step in-place kernel (variant 0) and kernels that
  * advance two steps per launch through LDS (1: 64x4 x 16, 2: 64x8 x 16,
  * 3: 64x8 x 32, 4: 64x4 x 32, 5: 32x8 x 16, 6: 64x12 x 16, 7: 64x12 x 32,

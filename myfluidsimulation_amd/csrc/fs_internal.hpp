@@ -165,12 +165,13 @@ struct HomeLaunch {
 };
 // HOME kernel variants: the plain per-cell gather, three tile shapes (64 x TY
 // columns, ZC-plane z-chunks) of the first tiled form and the same three of
-// the second (home_tile2), three of the third (home_tile3), and three of the
+// the second (home_tile2), three of the third (home_tile3), and five of the
 // two-step tile (home_pair_tile: two steps per launch, 64 x TY x ZC)
 constexpr int kHomePlain = 0, kHomeT4Z16 = 1, kHomeT8Z16 = 2, kHomeT8Z32 = 3, kHome2T4Z16 = 4,
               kHome2T8Z16 = 5, kHome2T8Z32 = 6, kHome2T16Z32 = 7, kHome2T8Z64 = 8,
               kHome3T4Z16 = 9, kHome3T8Z32 = 10, kHome3T8Z64 = 11, kHomeP11Z64 = 12,
-              kHomeP11Z32 = 13, kHomeP10Z64 = 14, kHomeVariants = 15;
+              kHomeP11Z32 = 13, kHomeP10Z64 = 14, kHomeP12Z64 = 15, kHomeP12Z32 = 16,
+              kHomeVariants = 17;
 bool home_variant_ok(const Geo& g, int variant);
 int home_default_variant(const Geo& g);
 // two steps per launch

@@ -17,7 +17,9 @@
 //   role  TY+2        E's two side columns (x0-1, x0+64), F's two side
 //                     columns (x0-2, x0+65), the four ends of F's rows and
 //                     F's four corners (the side wave)
-//   roles TY+3, TY+4  F's rows y0-2 and y0+TY+1
+//   roles TY+3, TY+4  F's rows y0-2 and y0+TY+1; where 64 x (TY+5) would pass
+//                     1024 threads (TY = 12), role TY+3 holds both rows, one
+//                     cell of each per lane
 // Each lane walks the source planes p = z0-2 .. z0+ZC+1; per plane:
 //   A  every lane emits m^t(p) of its cell into LDS plane set A (E cells and
 //      the side wave all 19 directions, F's rows the five that enter E);
@@ -57,6 +59,13 @@ struct PairLds {
   static constexpr int WA = 68, HA = TY + 4, PA = WA * HA;  // stage A: F, offset (2, 2)
   static constexpr int WC = 66, HC = TY + 2, PC = WC * HC;  // stage C: E, offset (1, 1)
   static constexpr int FLOATS = 16 * (PA + PC);
+  // F's two rows share one wave (two cells per lane) where a wave each would
+  // pass the 1024 threads of a workgroup
+  static constexpr bool FOLD = 64 * (TY + 5) > 1024;
+  static constexpr int WAVES = FOLD ? TY + 4 : TY + 5;
+  static constexpr int THREADS = 64 * WAVES;
+  static_assert(THREADS <= 1024, "one workgroup per tile");
+  static_assert(4 * (TY + 2) + 8 <= 64, "the side wave holds E's and F's columns and F's ends");
 };
 
 // Directions that enter the inner region from one side of a ring:
@@ -140,6 +149,13 @@ __device__ __forceinline__ int pair_wave_role(int hw) {
     // role:   13   1  2  3  14  5  6  7  8  9  10  11  4  0  12  15
     constexpr uint64_t kRole = 0xFC04BA98765E321DULL;
     return int((kRole >> (4 * hw)) & 15u);
+  } else if constexpr (TY == 12) {
+    // twelve tile rows, E's two ring rows, the side wave and the folded F
+    // rows: three tile rows and one of the other four roles per SIMD
+    // hw:      0   1   2   3  4  5 ... 15
+    // role:    0  13  14  15  1  2 ... 12
+    constexpr uint64_t kRole = 0xCBA987654321FED0ULL;
+    return int((kRole >> (4 * hw)) & 15u);
   } else {
     return hw;
   }
@@ -153,6 +169,18 @@ __device__ __forceinline__ int pair_wave_role(int hw) {
 struct PairRing {
   float Ao[5], Am[5], Br[9], Cb[5];
 };
+
+// A folded F-row lane gathers nothing, so its stage-D ring (24 registers it
+// never uses otherwise) holds the moments of its second cell: plane-parity
+// buffer B, moment k.  This keeps the two-row lanes within the registers of
+// the others.
+template <int B>
+__device__ __forceinline__ float& fold_slot(PairRing& R, int k) {
+  if constexpr (B == 0)
+    return k < 9 ? R.Br[k] : R.Ao[0];
+  else
+    return k < 5 ? R.Am[k] : R.Cb[k - 5];
+}
 
 template <int TY, int ZC, int BB>
 __device__ __forceinline__ void home_pair_tile(const Geo& g, const Phys& ph, const HomeLaunch& a,
@@ -217,6 +245,11 @@ __device__ __forceinline__ void home_pair_tile(const Geo& g, const Phys& ph, con
   const uint32_t cxy = uint32_t(gy) * g.nx + gx;
   const uint32_t cxy4 = cxy * 4u, plane4 = g.nxy * 4u;
   const int posA = (yl + 2) * PL::WA + (xl + 2);
+  // folded F rows: the lane's second cell (x0+l, y0+TYe+1), direction set 2
+  const bool fold = PL::FOLD && w == TY + 3 && hasCell;
+  const uint32_t cxy2_4 =
+      (uint32_t(((y0 + TYe + 1) % g.ny + g.ny) % g.ny) * g.nx + uint32_t(gx)) * 4u;
+  const int posA2 = (TYe + 3) * PL::WA + (xl + 2);
   const int posC = (yl + 1) * PL::WC + (xl + 1);
   const int last = z0 + ZCe + 1;  // the last source plane
 
@@ -242,6 +275,17 @@ __device__ __forceinline__ void home_pair_tile(const Geo& g, const Phys& ph, con
       for (int k = 0; k < 10; ++k) m.v[k] = ld_b(a.m + size_t(k) * P, off);
     }
   };
+  auto load_fold = [&](auto bc, int p) {
+    constexpr int B = decltype(bc)::value;
+    if constexpr (PL::FOLD) {
+      if (fold) {
+        const uint32_t off = plane_of(p) * plane4 + cxy2_4;
+#pragma unroll
+        for (int k = 0; k < 10; ++k) fold_slot<B>(RD, k) = ld_b(a.m + size_t(k) * P, off);
+      }
+    }
+  };
+  load_fold(std::integral_constant<int, 0>{}, z0 - 2);
   load_mom_plane(z0 - 2, Mm[0]);
   if (isE) flr = a.flags[plane_of(z0 - 2) * g.nxy + cxy];
 
@@ -273,8 +317,9 @@ __device__ __forceinline__ void home_pair_tile(const Geo& g, const Phys& ph, con
   // plane just emitted); r0, r5, r6: this lane's own column values of the
   // plane just emitted.  Returns the q-ordered sums of pOut's gathered
   // values (home_finish) in out, computed before the ring slots they come
-  // from are refilled.
-  auto gather_stage = [&](auto wc, auto psc, PairRing& R, const float* Lx, int pos,
+  // from are refilled.  need (the same for the whole workgroup): pOut is an
+  // output of this chunk; the first planes of a chunk only fill the ring.
+  auto gather_stage = [&](auto wc, auto psc, PairRing& R, const float* Lx, int pos, bool need,
                           bool anyNearOut, bool anyNearCur, uint32_t nbOut, uint32_t nbCur,
                           float r0, float r5, float r6, float (&out)[10]) {
     constexpr int W = decltype(wc)::value;
@@ -303,21 +348,26 @@ __device__ __forceinline__ void home_pair_tile(const Geo& g, const Phys& ph, con
     };
     // output pOut: e_z = +1 from the ring (two planes back), e_z = 0 from
     // the ring (one plane back), e_z = -1 from this plane
-    float f[19];
-    for_q<0, 19>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      if constexpr (CZ[q] > 0) f[q] = R.Ao[kSlot[q]];
-      if constexpr (CZ[q] == 0) f[q] = R.Br[kSlot[q]];
-      if constexpr (CZ[q] < 0) f[q] = lds_in(qc);
-    });
-    if (anyNearOut) {
-      for_q<1, 19>([&](auto qc) {
+    if (need) {
+      float f[19];
+      for_q<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        if constexpr (CZ[q] < 0)
-          if ((nbOut >> OPP[q]) & 1u) f[q] = R.Cb[kSlot[q]];
+        if constexpr (CZ[q] > 0) f[q] = R.Ao[kSlot[q]];
+        if constexpr (CZ[q] == 0) f[q] = R.Br[kSlot[q]];
+        if constexpr (CZ[q] < 0) f[q] = lds_in(qc);
       });
+      if (anyNearOut) {
+        for_q<1, 19>([&](auto qc) {
+          constexpr int q = decltype(qc)::value;
+          if constexpr (CZ[q] < 0)
+            if ((nbOut >> OPP[q]) & 1u) f[q] = R.Cb[kSlot[q]];
+        });
+      }
+      moments(f, out);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 10; ++k) out[k] = 0.0f;
     }
-    moments(f, out);
     // ring: this plane's e_z = 0 values (next output), e_z = +1 (the one
     // after); the +1 values of the next output move up
 #pragma unroll
@@ -379,6 +429,7 @@ __device__ __forceinline__ void home_pair_tile(const Geo& g, const Phys& ph, con
     if (isT && (fl2 & F_INLET)) uin2 = a.inlet[size_t(plane_of(o2)) * g.ny + gy];
     uint32_t flNext = 0u;
     if (p < last) {
+      load_fold(std::integral_constant<int, 1 - K>{}, p + 1);
       load_mom_plane(p + 1, Mm[1 - K]);
       if (isE) flNext = a.flags[plane_of(p + 1) * g.nxy + cxy];
     }
@@ -397,6 +448,15 @@ __device__ __forceinline__ void home_pair_tile(const Geo& g, const Phys& ph, con
       else
         emit_ring_set<PL::PA>(LA, posA, h, setA);
     }
+    if constexpr (PL::FOLD) {
+      if (fold) {
+        Mom m2;
+#pragma unroll
+        for (int k = 0; k < 10; ++k) m2.v[k] = fold_slot<K>(RD, k);
+        const HomeSrc h2(m2, omm);
+        emit_ring_set<PL::PA>(LA, posA2, h2, 2);
+      }
+    }
     __syncthreads();
 
     // ---- B: m^{t+1}(o1) on E ----
@@ -406,7 +466,7 @@ __device__ __forceinline__ void home_pair_tile(const Geo& g, const Phys& ph, con
     float m1[10];
     if (isE) {
       gather_stage(std::integral_constant<int, PL::WA>{}, std::integral_constant<int, PL::PA>{},
-                   RB, LA, posA, anyNear1, anyNearP, nb1, nb0, a0, a5, a6, m1);
+                   RB, LA, posA, o1 >= z0 - 1, anyNear1, anyNearP, nb1, nb0, a0, a5, a6, m1);
     } else {
 #pragma unroll
       for (int k = 0; k < 10; ++k) m1[k] = 0.0f;
@@ -435,7 +495,7 @@ __device__ __forceinline__ void home_pair_tile(const Geo& g, const Phys& ph, con
       float out[10];
       if (isT) {
         gather_stage(std::integral_constant<int, PL::WC>{}, std::integral_constant<int, PL::PC>{},
-                     RD, LC, posC, anyNear2, anyNearC, nb2, nb1, c0, c5, c6, out);
+                     RD, LC, posC, o2 >= z0, anyNear2, anyNearC, nb2, nb1, c0, c5, c6, out);
       } else {
 #pragma unroll
         for (int k = 0; k < 10; ++k) out[k] = 0.0f;

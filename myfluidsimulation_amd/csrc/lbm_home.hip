@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(TX * TY) k_home_tiled3(Geo g, Phys ph, HomeLau
 }
 
 template <int TY, int ZC, int BB>
-__global__ void __launch_bounds__(64 * (TY + 5)) k_home_pair(Geo g, Phys ph, HomeLaunch a) {
+__global__ void __launch_bounds__(PairLds<TY>::THREADS) k_home_pair(Geo g, Phys ph, HomeLaunch a) {
   __shared__ float lds[PairLds<TY>::FLOATS];
   const int tilesX = (g.nx + 63) / 64;
   // the XCD-contiguous tile order of k_home_tiled
@@ -113,10 +113,10 @@ hipError_t launch_pair(const Geo& g, const Phys& ph, const HomeLaunch& a, hipStr
   const unsigned blocks =
       unsigned(((g.nx + 63) / 64) * ((g.ny + TY - 1) / TY) * ((g.nz + ZC - 1) / ZC));
   if (ph.bounce == 0)
-    hipLaunchKernelGGL((k_home_pair<TY, ZC, 0>), dim3(blocks), dim3(64 * (TY + 5)), 0, st, g, ph,
+    hipLaunchKernelGGL((k_home_pair<TY, ZC, 0>), dim3(blocks), dim3(PairLds<TY>::THREADS), 0, st, g, ph,
                        a);
   else
-    hipLaunchKernelGGL((k_home_pair<TY, ZC, 1>), dim3(blocks), dim3(64 * (TY + 5)), 0, st, g, ph,
+    hipLaunchKernelGGL((k_home_pair<TY, ZC, 1>), dim3(blocks), dim3(PairLds<TY>::THREADS), 0, st, g, ph,
                        a);
   return hipGetLastError();
 }
@@ -215,14 +215,18 @@ int home_default_variant(const Geo& g) {
   const int v = home_one_step_default(g);
   if (v == kHomePlain || !home_variant_ok(g, kHomeP11Z64)) return v;
   // The two-step tile (1024-thread workgroups, one per CU) where it gives
-  // enough of them: 64x11 columns with 64-plane chunks from 2048 workgroups
-  // (512^3: 3008; 61.8 GLUPS against 57.7 for the third form), else 32-plane
-  // chunks from 2048 (768x256x256 channel: 2304; 56.7 against 53.6).  256^3
-  // (768 workgroups of 64x11x32) ties with the third form (57.8 against
-  // 57.5) and keeps it.
-  const long tiles11 = long((g.nx + 63) / 64) * ((g.ny + 10) / 11);
+  // enough of them (measured, two runs each):
+  //  * 64x12 columns with 64-plane chunks from 2048 workgroups (512^3: 2752;
+  //    71.3 GLUPS against 70.7 for 64x11x64);
+  //  * 64x11 columns with 64-plane chunks from 2048 workgroups;
+  //  * 64x11 columns with 32-plane chunks from 768 (768x256x256 channel: 2304,
+  //    64.4 GLUPS against 61.9 for 64x12x32; 256^3: 768, three full rounds,
+  //    65.6 against 57.0 for the third form and 62.0 for 64x12x32).
+  const long cols = (g.nx + 63) / 64;
+  const long tiles11 = cols * ((g.ny + 10) / 11), tiles12 = cols * ((g.ny + 11) / 12);
+  if (tiles12 * ((g.nz + 63) / 64) >= 2048) return kHomeP12Z64;
   if (tiles11 * ((g.nz + 63) / 64) >= 2048) return kHomeP11Z64;
-  if (tiles11 * ((g.nz + 31) / 32) >= 2048) return kHomeP11Z32;
+  if (tiles11 * ((g.nz + 31) / 32) >= 768) return kHomeP11Z32;
   return v;
 }
 
@@ -236,6 +240,8 @@ hipError_t launch_home_pair(const Geo& g, const Phys& ph, const HomeLaunch& a, i
     case kHomeP11Z64: return launch_pair<11, 64>(g, ph, a, st);
     case kHomeP11Z32: return launch_pair<11, 32>(g, ph, a, st);
     case kHomeP10Z64: return launch_pair<10, 64>(g, ph, a, st);
+    case kHomeP12Z64: return launch_pair<12, 64>(g, ph, a, st);
+    case kHomeP12Z32: return launch_pair<12, 32>(g, ph, a, st);
     default: break;
   }
   return hipErrorInvalidValue;

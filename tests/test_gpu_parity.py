@@ -214,7 +214,7 @@ def test_home_tile_shapes(fs, oracle, dims):
     inlet = np.full(ny * nz, 0.04, np.float32)
     fields = wl.uniform_flow(nx * ny * nz, 0.04)
     tiled = (nx - 1) % 64 != 0
-    for variant in (tuple(range(1, 15)) if tiled else (0,)):
+    for variant in (tuple(range(1, 17)) if tiled else (0,)):
         res = run_pair(fs, oracle, 1, (nx, ny, nz), 0.62, 9, flags=flags, fields=fields,
                        inlet=inlet, force=(1e-6, 0.0, 0.0), variant=variant)
         assert_exact(res, f"home tile {dims} variant {variant}")
@@ -238,7 +238,7 @@ def test_home_variants_and_autotune(fs, oracle):
     o.reinit(*fields)
     o.step(6)
     ref = o.state()
-    for variant in tuple(range(15)) + (-1,):
+    for variant in tuple(range(17)) + (-1,):
         with fs.Solver(1, nx, ny, nz, 0.58) as s:
             s.set_flags(flags)
             s.set_inlet_profile(inlet)
@@ -247,14 +247,15 @@ def test_home_variants_and_autotune(fs, oracle):
             if variant == -1:
                 before = s.state()
                 chosen = s.autotune(3)
-                assert chosen in range(15) and s.variant == chosen
+                assert chosen in range(17) and s.variant == chosen
                 np.testing.assert_array_equal(s.state(), before)
             s.step(6)
             np.testing.assert_array_equal(s.state(), ref)
 
 
-def test_home_pair_outlet_rules(fs, oracle):
-    """The two-step HOME tile (variant 12) where an outlet sits on a tile's
+@pytest.mark.parametrize("variant", [12, 15], ids=["64x11", "64x12"])
+def test_home_pair_outlet_rules(fs, oracle, variant):
+    """The two-step HOME tile (variants 12 and 15) where an outlet sits on a tile's
     left ring column: a channel whose outlet column x = nx-1 (nx % 64 == 0)
     wraps onto the inlet keeps the two-step kernel; an outlet at x = 63 whose
     downstream cells at x = 64 are fluid sends the solver to a one-step
@@ -274,8 +275,8 @@ def test_home_pair_outlet_rules(fs, oracle):
                 s.set_flags(fl)
                 s.set_inlet_profile(inlet)
                 s.set_bounce_rule(rule)
-                s.set_variant(12)
-                assert (s.variant == 12) == pair
+                s.set_variant(variant)
+                assert (s.variant == variant) == pair
                 s.reinit_from_macro(*fields)
                 o = oracle.Oracle(1, nx, ny, nz, 0.6)
                 o.set_flags(fl)
