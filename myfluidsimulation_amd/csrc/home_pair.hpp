@@ -161,13 +161,14 @@ __device__ __forceinline__ int pair_wave_role(int hw) {
   }
 }
 
-// Gather rings of one stage (B or D) for one lane: values with e_z = +1 (Ao
-// for the output due now, Am for the next one), e_z = 0 for the next output
-// (Br), and the own values that replace e_z = -1 values of the next output of
-// a near-wall cell (Cb).  Each slot is read for the output due before it is
-// refilled for the next.
+// Gather rings of one stage (B or D) for one lane: values with e_z = +1 by
+// the parity of the plane that sent them (A[K]: read by the output two planes
+// later, which has the same parity, so no value moves between slots), e_z = 0
+// for the next output (Br), and the own values that replace e_z = -1 values
+// of the next output of a near-wall cell (Cb).  Each slot is read for the
+// output due before it is refilled.
 struct PairRing {
-  float Ao[5], Am[5], Br[9], Cb[5];
+  float A[2][5], Br[9], Cb[5];
 };
 
 // A folded F-row lane gathers nothing, so its stage-D ring (24 registers it
@@ -177,9 +178,9 @@ struct PairRing {
 template <int B>
 __device__ __forceinline__ float& fold_slot(PairRing& R, int k) {
   if constexpr (B == 0)
-    return k < 9 ? R.Br[k] : R.Ao[0];
+    return k < 9 ? R.Br[k] : R.A[0][0];
   else
-    return k < 5 ? R.Am[k] : R.Cb[k - 5];
+    return k < 5 ? R.A[1][k] : R.Cb[k - 5];
 }
 
 template <int TY, int ZC, int BB>
@@ -260,7 +261,8 @@ __device__ __forceinline__ void home_pair_tile(const Geo& g, const Phys& ph, con
     for (int k = 0; k < 10; ++k) Mm[s].v[k] = 0.0f;
   PairRing RB, RD;
 #pragma unroll
-  for (int k = 0; k < 5; ++k) RB.Ao[k] = RB.Am[k] = RD.Ao[k] = RD.Am[k] = RB.Cb[k] = RD.Cb[k] = 0.0f;
+  for (int k = 0; k < 5; ++k)
+    RB.A[0][k] = RB.A[1][k] = RD.A[0][k] = RD.A[1][k] = RB.Cb[k] = RD.Cb[k] = 0.0f;
 #pragma unroll
   for (int k = 0; k < 9; ++k) RB.Br[k] = RD.Br[k] = 0.0f;
   // flags of planes p (byte 0), p-1 (byte 1), p-2 (byte 2) of E cells;
@@ -317,11 +319,14 @@ __device__ __forceinline__ void home_pair_tile(const Geo& g, const Phys& ph, con
   // plane just emitted); r0, r5, r6: this lane's own column values of the
   // plane just emitted.  Returns the q-ordered sums of pOut's gathered
   // values (home_finish) in out, computed before the ring slots they come
-  // from are refilled.  need (the same for the whole workgroup): pOut is an
-  // output of this chunk; the first planes of a chunk only fill the ring.
-  auto gather_stage = [&](auto wc, auto psc, PairRing& R, const float* Lx, int pos, bool need,
+  // from are refilled.  K: the parity of this plane.  need (the same for the
+  // whole workgroup): pOut is an output of this chunk; the first planes of a
+  // chunk only fill the ring, and out is left unset.
+  auto gather_stage = [&](auto kc, auto wc, auto psc, PairRing& R, const float* Lx, int pos,
+                          bool need,
                           bool anyNearOut, bool anyNearCur, uint32_t nbOut, uint32_t nbCur,
                           float r0, float r5, float r6, float (&out)[10]) {
+    constexpr int K = decltype(kc)::value;
     constexpr int W = decltype(wc)::value;
     constexpr int PS = decltype(psc)::value;
     auto lds_in = [&](auto qc) {  // value arriving along q from c - e_q (in-plane part)
@@ -352,7 +357,7 @@ __device__ __forceinline__ void home_pair_tile(const Geo& g, const Phys& ph, con
       float f[19];
       for_q<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        if constexpr (CZ[q] > 0) f[q] = R.Ao[kSlot[q]];
+        if constexpr (CZ[q] > 0) f[q] = R.A[K][kSlot[q]];
         if constexpr (CZ[q] == 0) f[q] = R.Br[kSlot[q]];
         if constexpr (CZ[q] < 0) f[q] = lds_in(qc);
       });
@@ -364,17 +369,12 @@ __device__ __forceinline__ void home_pair_tile(const Geo& g, const Phys& ph, con
         });
       }
       moments(f, out);
-    } else {
-#pragma unroll
-      for (int k = 0; k < 10; ++k) out[k] = 0.0f;
     }
     // ring: this plane's e_z = 0 values (next output), e_z = +1 (the one
-    // after); the +1 values of the next output move up
-#pragma unroll
-    for (int k = 0; k < 5; ++k) R.Ao[k] = R.Am[k];
+    // after, same parity)
     for_q<0, 19>([&](auto qc) {
       constexpr int q = decltype(qc)::value;
-      if constexpr (CZ[q] > 0) R.Am[kSlot[q]] = lds_in(qc);
+      if constexpr (CZ[q] > 0) R.A[K][kSlot[q]] = lds_in(qc);
       if constexpr (CZ[q] == 0) R.Br[kSlot[q]] = lds_in(qc);
     });
     // the next output (this plane) of a near-wall cell: where c - e_q is
@@ -386,7 +386,7 @@ __device__ __forceinline__ void home_pair_tile(const Geo& g, const Phys& ph, con
         constexpr int bq = BB == 0 ? OPP[q] : OPP_REF[q];
         if ((nbCur >> OPP[q]) & 1u) {
           const float v = own(std::integral_constant<int, bq>{});
-          if constexpr (CZ[q] > 0) R.Ao[kSlot[q]] = v;
+          if constexpr (CZ[q] > 0) R.A[1 - K][kSlot[q]] = v;
           if constexpr (CZ[q] == 0) R.Br[kSlot[q]] = v;
           if constexpr (CZ[q] < 0) R.Cb[kSlot[q]] = v;
         }
@@ -463,13 +463,13 @@ __device__ __forceinline__ void home_pair_tile(const Geo& g, const Phys& ph, con
     const bool finB = isE && o1 >= z0 - 1 && o1 <= z0 + ZCe;
     const bool anyNear1 = __ballot(isE && (fl1 & F_NEAR)) != 0;
     const bool anyNearP = __ballot(isE && (flP & F_NEAR)) != 0;
+    // m1 and out below are set only where a lane has an output (isE / isT
+    // and need); nothing reads them elsewhere, so they get no default
     float m1[10];
     if (isE) {
-      gather_stage(std::integral_constant<int, PL::WA>{}, std::integral_constant<int, PL::PA>{},
-                   RB, LA, posA, o1 >= z0 - 1, anyNear1, anyNearP, nb1, nb0, a0, a5, a6, m1);
-    } else {
-#pragma unroll
-      for (int k = 0; k < 10; ++k) m1[k] = 0.0f;
+      gather_stage(kc, std::integral_constant<int, PL::WA>{},
+                   std::integral_constant<int, PL::PA>{}, RB, LA, posA, o1 >= z0 - 1, anyNear1,
+                   anyNearP, nb1, nb0, a0, a5, a6, m1);
     }
     finish_cell(finB, fl1, uin1, m1);
 
@@ -494,11 +494,9 @@ __device__ __forceinline__ void home_pair_tile(const Geo& g, const Phys& ph, con
       const bool anyNearC = __ballot(isT && (fl1 & F_NEAR)) != 0;
       float out[10];
       if (isT) {
-        gather_stage(std::integral_constant<int, PL::WC>{}, std::integral_constant<int, PL::PC>{},
-                     RD, LC, posC, o2 >= z0, anyNear2, anyNearC, nb2, nb1, c0, c5, c6, out);
-      } else {
-#pragma unroll
-        for (int k = 0; k < 10; ++k) out[k] = 0.0f;
+        gather_stage(kc, std::integral_constant<int, PL::WC>{},
+                     std::integral_constant<int, PL::PC>{}, RD, LC, posC, o2 >= z0, anyNear2,
+                     anyNearC, nb2, nb1, c0, c5, c6, out);
       }
       finish_cell(finD, fl2, uin2, out);
       if (finD) {
