@@ -3,8 +3,9 @@
 The reference voxelizer needs GLM, which this image lacks, so it cannot be
 built here; the restatement is pinned by known answers instead: closed-form
 voxel counts of axis-aligned boxes, an independent double-precision
-separating-axis test away from ties, sphere volume, STL byte fixtures, and
-a brute-force taxicab distance for the band.
+separating-axis test away from ties, an exact rational one on dyadic inputs
+(ties included), sphere volume, STL byte fixtures, and a brute-force taxicab
+distance for the band.
 """
 import os
 import struct
@@ -64,6 +65,68 @@ def test_sat_against_float64_sat(oracle):
         assert oracle.tri_box_overlap(c, h, *tri) == ok64
         checked += 1
     assert checked > 3000
+
+
+def sat_exact(c, h, v0, v1, v2):
+    """Triangle / axis-aligned box overlap by the separating axis theorem in
+    exact rational arithmetic, with touching counted as overlap (separation
+    only where a projection gap is strictly positive, as voxelize.cpp:23-81
+    compares); returns (overlap, tie) where tie says some axis touched
+    exactly (gap 0).  Axes with a zero edge or a zero cross product are
+    skipped, as the reference's 1e-12 / 1e-20 thresholds skip them."""
+    from fractions import Fraction as Fr
+
+    c = [Fr(x) for x in c]
+    h = [Fr(x) for x in h]
+    v = [[Fr(a) - b for a, b in zip(p, c)] for p in (v0, v1, v2)]
+
+    def cross(a, b):
+        return [a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]]
+
+    def dot(a, b):
+        return sum(x * y for x, y in zip(a, b))
+
+    e = [[v[1][i] - v[0][i] for i in range(3)], [v[2][i] - v[1][i] for i in range(3)],
+         [v[0][i] - v[2][i] for i in range(3)]]
+    axes = [[Fr(int(i == j)) for j in range(3)] for i in range(3)]
+    axes.append(cross(e[0], e[1]))
+    for ed in e:
+        if all(x == 0 for x in ed):
+            continue
+        for i in range(3):
+            axes.append(cross(ed, [Fr(int(i == j)) for j in range(3)]))
+    tie = False
+    for k, a in enumerate(axes):
+        if k > 3 and dot(a, a) == 0:
+            continue
+        p = [dot(a, vi) for vi in v]
+        r = sum(abs(x) * y for x, y in zip(a, h))
+        if k == 3:  # the face normal: |n . v0| against r
+            gap = abs(p[0]) - r
+        else:
+            gap = max(min(p) - r, -r - max(p))
+        tie = tie or gap == 0
+        if gap > 0:
+            return False, tie
+    return True, tie
+
+
+def test_sat_exact_ties_dyadic(oracle):
+    """Dyadic vertices (multiples of 1/4 in [-2, 2]) and boxes (centres on the
+    half grid, half-sizes 1/4 or 1/2) keep every float32 product and sum of
+    the test exact, so the oracle must give the exact answer, ties included:
+    a vertex, edge or face exactly on a box face counts as overlap.  A strict
+    / non-strict comparison swapped on any axis fails here."""
+    rng = np.random.default_rng(11)
+    ties = 0
+    for _ in range(8000):
+        c = rng.integers(-2, 3, 3) / 2.0
+        h = rng.choice([0.25, 0.5], 3)
+        tri = rng.integers(-8, 9, (3, 3)) / 4.0
+        ok, tie = sat_exact(c, h, *tri)
+        assert oracle.tri_box_overlap(c, h, *tri) == ok, (c, h, tri)
+        ties += tie and ok
+    assert ties > 150
 
 
 @pytest.mark.parametrize("n,lo,hi,expect", [
