@@ -8,7 +8,8 @@
   kinetic energy follows the analytic decay (u0^2/8) exp(-6 nu k^2 t), and
   the mirror symmetries of the vortex (u odd and v even under x -> -x, the
   reverse under y -> -y, both even under z -> -z; the w the nonlinear term
-  generates odd under z -> -z, even under x and y) hold.
+  generates odd under z -> -z, even under x and y) hold; after 4 steps both
+  equal the FP32 oracle bit for bit (fields and full state).
 * The 768x256x256 channel's icosphere (R=40): the voxelized flag grid equals
   the oracle voxelizer's and its band (d0 = 1, 2, 4) the oracle band; BGK,
   HOME and Hybrid with bands 1, 2, 4 on that channel equal the FP32 oracle bit
@@ -99,6 +100,29 @@ def test_tgv512_properties(fs, method):
     assert np.abs(w + fz(w)).max() < tol and np.abs(w - fx(w)).max() < tol
     assert np.abs(w - fy(w)).max() < tol and float(np.abs(w).max()) > 1e-5
     assert np.abs(r - fx(r)).max() < 5e-6 and np.abs(r - fz(r)).max() < 5e-6
+
+
+@pytest.mark.parametrize("method", [0, 1], ids=["bgk", "home"])
+def test_tgv512_exact_short(fs, oracle, method):
+    """The 512^3 vortex's default kernels (two steps per launch: BGK 64x12x64
+    tiles, HOME the 64x12x64 tile with its folded ring rows) equal the FP32
+    oracle bit for bit after 4 steps (fields and full state)."""
+    n, tau, u0, steps = 512, 0.6, 0.05, 4
+    print(f"tgv512 exact method {method}", flush=True)
+    fields = wl.taylor_green_3d(n, n, n, u0)
+    s = fs.Solver(method, n, n, n, tau)
+    assert s.variant == (8 if method == 0 else 15)
+    s.reinit_from_macro(*fields)
+    s.step(steps)
+    o = oracle.Oracle(method, n, n, n, tau)
+    o.reinit(*fields)
+    del fields
+    o.step(steps)
+    for k in range(4):
+        assert np.array_equal(s.macro(k), o.macro(k)), f"macro {k}"
+    assert np.array_equal(s.state(), o.state())
+    s.close()
+    o.close()
 
 
 def test_channel768_geometry_exact(fs, oracle, tmp_path):
