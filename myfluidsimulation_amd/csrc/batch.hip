@@ -13,7 +13,7 @@
 // is a multiple of 64, else 32 x 8 tiles, which waste fewer lanes than
 // 64-wide rows on the sweep's other edges of 48..112 cells), or one
 // (k_batch_pop) for an odd step count.  HOME domains use the third tiled
-// form (home_tile3) on the same two tile shapes.
+// form (home_tile3) on the same two tile shapes (32-plane chunks).
 #include "home_body.hpp"
 #include "pair_body.hpp"
 #include "pop_body.hpp"
@@ -22,7 +22,11 @@ namespace fs {
 
 namespace {
 
-constexpr int kBatchHomeTX = 32, kBatchHomeTY = 8, kBatchHomeZC = 16;
+// HOME tiles walk 32-plane chunks, population tiles 16-plane ones (seeded
+// 64-domain sweep: HOME chunks of 8 / 16 / 32 / 64 / 128 planes 42.1 / 42.9 /
+// 43.3 / 41.2 / 39.9 GLUPS with 16-plane population chunks; population
+// chunks of 24 or 32 planes 41.4-41.9 / 41.0 with 32-plane HOME chunks)
+constexpr int kBatchHomeTX = 32, kBatchHomeTY = 8, kBatchHomeZC = 32;
 constexpr int kBatchHomeWideTX = 64, kBatchHomeWideTY = 4;  // nx a multiple of 64
 constexpr int kBatchHomeLds =
     16 * (TileLds16<kBatchHomeWideTX, kBatchHomeWideTY>::PL >
